@@ -1,0 +1,138 @@
+/*
+ * sweep_amd.h -- the C boundary of libswp_core.so (MI355X / gfx950).
+ *
+ * The Python host (sweep_amd.propagator) calls these through ctypes; any
+ * other host can bind them the same way.  Plain C: no torch types, plain
+ * pointers and sizes.  Errors come back as a non-zero return with the message
+ * copied into ``err``; nothing is thrown across.  The core never allocates:
+ * the caller owns every buffer, including the per-call workspace and the
+ * forward->backward tape, whose byte sizes it asks for with swp_query.
+ *
+ * Each entry point below replaces a function of the reference's C ABI
+ * (src/sweep/csrc/core/capi.h of DeepWave-KAUST/sweep); the citation sits on
+ * the declaration.  The input structs are this project's own: one
+ * swp_problem (grid, models, coefficients, geometry, memory strategy, caller
+ * buffers) in place of the reference's ForwardInputCore / BackwardInputCore
+ * (csrc/core/input_core.h:10-132), whose wavefield lists the core here carves
+ * out of the caller's workspace instead.
+ */
+#ifndef SWEEP_AMD_H
+#define SWEEP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define SWP_CAPI __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWP_ABI_VERSION 1
+
+/* Memory strategy of a gradient run (reference: propagator/options.py
+ * Full / BoundarySaving(storage='gpu') / Ckpt(mode='chunk'|'recursive')). */
+enum {
+    SWP_MEM_NONE = 0,      /* forward only: nothing kept for a backward */
+    SWP_MEM_FULL = 1,      /* per-step imaging field (nt, B, grid) */
+    SWP_MEM_BOUNDARY = 2,  /* boundary strips + the last two states */
+    SWP_MEM_CKPT = 3,      /* state snapshots every ckpt_interval steps, chunk replay */
+};
+
+/* One propagation problem on the runtime grid.  Arrays are row-major with x
+ * fastest; a per-shot array is (B, nz[, ny], nx).  Coordinates are runtime
+ * cell indices in (x[, y], z) order, like the reference's sources_loc /
+ * receivers_loc.  Profiles follow the reference's pml_vals order:
+ *   cpmlr (acoustic): az, bz, dbz, [ay, by, dby,] ax, bx, dbx
+ *   cpmls (elastic):  az, bz, azh, bzh, ax, bx, axh, bxh
+ * Every pointer is device memory except lap_coes / grad_coes, which the core
+ * reads on the host (the kernels take the weights by value).                */
+typedef struct swp_problem {
+    int32_t ndim;            /* 2 or 3 */
+    int32_t nz, ny, nx;      /* runtime grid (ny = 1 in 2-D) */
+    int32_t B;               /* shots advanced together */
+    int32_t M;               /* half stencil order (spatial_order / 2) */
+    int32_t nt;              /* time steps */
+    int32_t abcn;            /* absorbing width (reconstruction seed rim = abcn + M) */
+    int32_t pad[6];          /* PML pad per face: z_lo, z_hi, y_lo, y_hi, x_lo, x_hi */
+    float dt;
+    float dz, dy, dx;
+    const float *lap_coes;   /* HOST (M+1): [c0, c1..cM], zero-sum float32 second derivative */
+    const float *grad_coes;  /* HOST (M+1): [0, c1..cM] centred first derivative */
+    const float *pml[12];
+    const float *models[3];  /* acoustic: vp; elastic: vp, vs, rho */
+    int32_t model_batched;   /* 1: every model is (B, grid); 0: one shared grid */
+    int32_t nsrc, nrec;
+    const float *source;            /* (B, nsrc, nt) forward source wavelets */
+    const int32_t *sources_loc;     /* (B, nsrc, ndim) */
+    const int32_t *receivers_loc;   /* (B, nrec, ndim) */
+    int32_t memory;          /* SWP_MEM_* */
+    int32_t ckpt_interval;   /* SWP_MEM_CKPT: steps between snapshots */
+    int32_t flags;           /* bit 0: accumulate illumination in the backward */
+    int32_t reserved;
+    void *tape;              /* forward -> backward state, swp_query's tape_bytes */
+    int64_t tape_bytes;
+    void *workspace;         /* per-call scratch, swp_query's workspace_bytes */
+    int64_t workspace_bytes;
+} swp_problem;
+
+/* Outputs of a forward entry (kind 0). */
+typedef struct swp_forward_io {
+    float *record;           /* (B, nrec, nt), time innermost (reference raw layout) */
+} swp_forward_io;
+
+/* Inputs/outputs of a backward entry (kind 1).  Gradients accumulate. */
+typedef struct swp_backward_io {
+    const float *adjoint_source;  /* (B, nrec, nt) residual, raw record layout */
+    float *grad_source;           /* (B, nsrc, nt) or NULL */
+    float *grad_models[3];        /* (B, grid) per model, or NULL */
+    float *illum_source;          /* (B, grid) sum_t u_tt^2, or NULL */
+    float *illum_receiver;        /* (B, grid) sum_t lambda^2, or NULL */
+} swp_backward_io;
+
+/* Replaces sweep_core_abi_version (capi.h:22-23). */
+SWP_CAPI int swp_core_abi_version(void);
+
+/* Replaces sweep_layout_count / sweep_layout_probe (capi.h:25-30): sizeof and
+ * offsetof of every struct field above, in declaration order, so the ctypes
+ * mirror can refuse a drifted build.  -1 past the end. */
+SWP_CAPI int64_t swp_layout_count(void);
+SWP_CAPI int64_t swp_layout_probe(int64_t i);
+
+/* Replace sweep_entry_count / _name / _kind / _available (capi.h:32-64).
+ * Names are <C_NAME>_<op>: forward, backward, backward_bs, backward_ckpt,
+ * backward_recursive_ckpt -- the callables equation._C() resolves. */
+SWP_CAPI int swp_entry_count(void);
+SWP_CAPI const char *swp_entry_name(int entry);
+SWP_CAPI int swp_entry_kind(int entry);
+SWP_CAPI int swp_entry_available(int entry);
+
+/* Replaces sweep_core_device (capi.h:66-69): "cuda", the torch device type of
+ * a ROCm GPU. */
+SWP_CAPI const char *swp_core_device(void);
+
+/* Workspace and tape bytes the entry needs for this problem (no reference
+ * counterpart: the reference's Python sizes every buffer from cuda_layout). */
+SWP_CAPI int swp_query(int entry, const swp_problem *in, int64_t *workspace_bytes,
+                       int64_t *tape_bytes, char *err, int err_cap);
+
+/* Replaces sweep_call (capi.h:71-72): io is swp_forward_io (kind 0) or
+ * swp_backward_io (kind 1).  Launches on the thread's stream, asynchronously;
+ * the caller keeps every buffer alive until that stream reaches the end. */
+SWP_CAPI int swp_call(int entry, const swp_problem *in, void *io, char *err, int err_cap);
+
+/* Replace sweep_set_stream / sweep_get_stream (capi.h:250-253): the
+ * thread-local hipStream_t every launch of this thread goes on. */
+SWP_CAPI void swp_set_stream(void *stream);
+SWP_CAPI void *swp_get_stream(void);
+
+/* Wall time of the last call's launches on the device, in ms, measured with
+ * HIP events on the call's stream (0 if timing is off).  No reference
+ * counterpart: the profiling hook of bench.py. */
+SWP_CAPI void swp_set_timing(int on);
+SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

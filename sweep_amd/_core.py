@@ -1,0 +1,186 @@
+"""ctypes front end of ``libswp_core.so`` (the C ABI in ``include/sweep_amd.h``).
+
+Plays the part of the reference's ``sweep._C`` / ``backend/c/loader.py``:
+dlopen once under a lock, refuse a library whose ABI version or struct layout
+differs from this mirror (``loader.py:199-228``), set the thread's stream from
+torch's current stream around each call (``backend/c/entries.py:89-102``),
+and turn a non-zero return into ``RuntimeError(message)`` (``loader.py:273-277``).
+
+There is no fallback: without the built library every entry raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libswp_core.so"
+ABI_VERSION = 1
+
+_F = ctypes.POINTER(ctypes.c_float)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+
+MEM_NONE, MEM_FULL, MEM_BOUNDARY, MEM_CKPT = 0, 1, 2, 3
+
+
+class SwpProblem(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int32), ("nz", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("nx", ctypes.c_int32), ("B", ctypes.c_int32), ("M", ctypes.c_int32),
+        ("nt", ctypes.c_int32), ("abcn", ctypes.c_int32), ("pad", ctypes.c_int32 * 6),
+        ("dt", ctypes.c_float), ("dz", ctypes.c_float), ("dy", ctypes.c_float),
+        ("dx", ctypes.c_float), ("lap_coes", _F), ("grad_coes", _F), ("pml", _F * 12),
+        ("models", _F * 3), ("model_batched", ctypes.c_int32), ("nsrc", ctypes.c_int32),
+        ("nrec", ctypes.c_int32), ("source", _F), ("sources_loc", _I32),
+        ("receivers_loc", _I32), ("memory", ctypes.c_int32), ("ckpt_interval", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32), ("tape", ctypes.c_void_p),
+        ("tape_bytes", ctypes.c_int64), ("workspace", ctypes.c_void_p),
+        ("workspace_bytes", ctypes.c_int64),
+    ]
+
+
+class SwpForwardIO(ctypes.Structure):
+    _fields_ = [("record", _F)]
+
+
+class SwpBackwardIO(ctypes.Structure):
+    _fields_ = [("adjoint_source", _F), ("grad_source", _F), ("grad_models", _F * 3),
+                ("illum_source", _F), ("illum_receiver", _F)]
+
+
+def _layout_table():
+    """sizeof / offsetof in the order swp_layout_probe reports them."""
+    out = [ctypes.sizeof(SwpProblem)]
+    out += [getattr(SwpProblem, name).offset for name, _ in SwpProblem._fields_]
+    out += [ctypes.sizeof(SwpForwardIO), SwpForwardIO.record.offset]
+    out += [ctypes.sizeof(SwpBackwardIO)]
+    out += [getattr(SwpBackwardIO, name).offset for name, _ in SwpBackwardIO._fields_]
+    return out
+
+
+_lock = threading.RLock()
+_lib = None
+_entries = None
+
+EXPORTS = (
+    "swp_core_abi_version", "swp_layout_count", "swp_layout_probe", "swp_entry_count",
+    "swp_entry_name", "swp_entry_kind", "swp_entry_available", "swp_core_device", "swp_query",
+    "swp_call", "swp_set_stream", "swp_get_stream", "swp_set_timing", "swp_last_kernel_stats",
+)
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def _declare(L):
+    L.swp_core_abi_version.restype = ctypes.c_int
+    L.swp_layout_count.restype = ctypes.c_int64
+    L.swp_layout_probe.restype = ctypes.c_int64
+    L.swp_layout_probe.argtypes = [ctypes.c_int64]
+    L.swp_entry_count.restype = ctypes.c_int
+    L.swp_entry_name.restype = ctypes.c_char_p
+    L.swp_entry_name.argtypes = [ctypes.c_int]
+    L.swp_entry_kind.restype = ctypes.c_int
+    L.swp_entry_kind.argtypes = [ctypes.c_int]
+    L.swp_entry_available.restype = ctypes.c_int
+    L.swp_entry_available.argtypes = [ctypes.c_int]
+    L.swp_core_device.restype = ctypes.c_char_p
+    L.swp_query.restype = ctypes.c_int
+    L.swp_query.argtypes = [ctypes.c_int, ctypes.POINTER(SwpProblem), ctypes.POINTER(ctypes.c_int64),
+                            ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p, ctypes.c_int]
+    L.swp_call.restype = ctypes.c_int
+    L.swp_call.argtypes = [ctypes.c_int, ctypes.POINTER(SwpProblem), ctypes.c_void_p,
+                           ctypes.c_char_p, ctypes.c_int]
+    L.swp_set_stream.restype = None
+    L.swp_set_stream.argtypes = [ctypes.c_void_p]
+    L.swp_get_stream.restype = ctypes.c_void_p
+    L.swp_set_timing.restype = None
+    L.swp_set_timing.argtypes = [ctypes.c_int]
+    L.swp_last_kernel_stats.restype = ctypes.c_int
+    L.swp_last_kernel_stats.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int64)]
+
+
+def load():
+    """The loaded library, checked; raises when it is missing or drifted."""
+    global _lib, _entries
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not _LIB_PATH.exists():
+            raise RuntimeError(
+                f"libswp_core is not built ({_LIB_PATH}); run `python -m sweep_amd.build` "
+                "(or __graft_entry__.build()). There is no fallback implementation.")
+        L = ctypes.CDLL(str(_LIB_PATH))
+        _declare(L)
+        v = L.swp_core_abi_version()
+        if v != ABI_VERSION:
+            raise RuntimeError(f"libswp_core ABI {v} != the Python mirror's {ABI_VERSION}; rebuild")
+        want = _layout_table()
+        n = L.swp_layout_count()
+        got = [L.swp_layout_probe(i) for i in range(n)]
+        if got != want:
+            raise RuntimeError(f"libswp_core struct layout drifted: core {got} vs mirror {want}; rebuild")
+        _entries = {L.swp_entry_name(i).decode(): i for i in range(L.swp_entry_count())}
+        _lib = L
+        return L
+
+
+def entries():
+    load()
+    return dict(_entries)
+
+
+def entry_id(name: str) -> int:
+    e = entries()
+    if name not in e:
+        raise RuntimeError(f"libswp_core has no entry {name!r}")
+    return e[name]
+
+
+def entry_available(name: str) -> bool:
+    L = load()
+    return L.swp_entry_available(entry_id(name)) == 1
+
+
+def _raise(rc, err, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {err.value.decode(errors='replace')}")
+
+
+def query(name: str, problem: SwpProblem):
+    L = load()
+    ws = ctypes.c_int64(0)
+    tp = ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(4096)
+    rc = L.swp_query(entry_id(name), ctypes.byref(problem), ctypes.byref(ws), ctypes.byref(tp), err, 4096)
+    _raise(rc, err, name)
+    return int(ws.value), int(tp.value)
+
+
+def call(name: str, problem: SwpProblem, io, device=None):
+    """Run one entry on torch's current stream of ``device``."""
+    import torch
+    L = load()
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    err = ctypes.create_string_buffer(4096)
+    with torch.cuda.device(dev):
+        L.swp_set_stream(ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+        rc = L.swp_call(entry_id(name), ctypes.byref(problem), ctypes.byref(io), err, 4096)
+    _raise(rc, err, name)
+
+
+def set_timing(on: bool):
+    load().swp_set_timing(1 if on else 0)
+
+
+def kernel_stats(which: int):
+    """(total ms, launches) of the last call's kernel class ``which``
+    (0 forward step, 1 adjoint step, 2 reconstruction, 3 other)."""
+    L = load()
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    if L.swp_last_kernel_stats(which, ctypes.byref(ms), ctypes.byref(n)) != 0:
+        raise RuntimeError("swp_last_kernel_stats failed")
+    return float(ms.value), int(n.value)

@@ -1,0 +1,70 @@
+// 2-D acoustic path: geometry, coefficient block and driver entry points.
+#pragma once
+
+#include "common.h"
+
+namespace swp {
+namespace a2d {
+
+constexpr int kMaxM = 16;
+
+// Runtime-grid geometry of one call, passed to every kernel by value.
+struct Geo {
+    int nz, nx, B, M, nt;
+    int pzl, pzh, pxl, pxh;    // PML pad per face
+    int X0, X1, Z0, Z1;        // physical box [pad_lo + M, n - pad_hi - M)
+    int abcn;
+    int W;                     // boundary strip width M + 1
+    int nxp, nzp;              // physical box extents
+    int64_t n;                 // cells per shot
+    int64_t mstride;           // model stride per shot (0: shared model)
+    int64_t ns;                // strip cells per shot per step
+    float dt, dx, dz;
+    float inv_dx2, inv_dz2, inv_dx, inv_dz, dt2, inv_dt2;
+};
+
+// FD weights: the fixed-order forward constants and the caller's tables.
+struct Coef {
+    float slap[4];
+    float slap_c0;
+    float sgrad[4];
+    float lap[kMaxM + 1];
+    float grad[kMaxM + 1];
+};
+
+struct Prof {
+    const float *az, *bz, *dbz, *ax, *bx, *dbx;
+};
+
+struct StepFields {
+    const float *vp;
+    const float *u_prev, *u_now;
+    float *u_next;
+    const float *psix, *psiz;
+    float *psixn, *psizn;
+    float *zetax, *zetaz;
+};
+
+struct AdjFields {
+    const float *vp;
+    const float *l_prev, *l_now;
+    float *l_next;
+    const float *psix, *psiz, *zetax, *zetaz;
+    float *psixo, *psizo, *zetaxo, *zetazo;
+};
+
+struct Ctx {
+    Geo g;
+    Coef c;
+    Prof pr;
+};
+
+Geo make_geo(const swp_problem &p);
+Ctx make_ctx(const swp_problem &p);
+int64_t workspace_bytes(const swp_problem &p, int kind);
+int64_t tape_bytes(const swp_problem &p);
+void forward(const Ctx &x, const swp_problem &p, float *record);
+void backward(const Ctx &x, const swp_problem &p, const swp_backward_io &io);
+
+}  // namespace a2d
+}  // namespace swp

@@ -1,0 +1,226 @@
+// The C boundary of libswp_core (include/sweep_amd.h): entry table,
+// dispatch, stream, layout probe, timing.  Every function catches everything
+// and answers with a return code plus a message, like the reference's
+// guarded() (csrc/cuda/common/capi.cu:230-242).
+#include <cstddef>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "acoustic2d.h"
+#include "acoustic3d.h"
+#include "elastic2d.h"
+
+namespace swp {
+
+namespace {
+thread_local hipStream_t t_stream = nullptr;
+
+struct Entry {
+    const char *name;
+    int kind;        // 0 forward, 1 backward
+    int equation;    // 0 acoustic2d, 1 acoustic3d, 2 elastic2d
+    int memory;      // the memory mode a backward entry serves (0: forward)
+};
+
+// <C_NAME>_<op>, the names equation._C() resolves (equations/base.py:295-324).
+const Entry kEntries[] = {
+    {"acoustic2d_forward", 0, 0, 0},
+    {"acoustic2d_backward", 1, 0, SWP_MEM_FULL},
+    {"acoustic2d_backward_bs", 1, 0, SWP_MEM_BOUNDARY},
+    {"acoustic2d_backward_ckpt", 1, 0, SWP_MEM_CKPT},
+    {"acoustic2d_backward_recursive_ckpt", 1, 0, SWP_MEM_CKPT},
+    {"acoustic3d_forward", 0, 1, 0},
+    {"acoustic3d_backward", 1, 1, SWP_MEM_FULL},
+    {"acoustic3d_backward_bs", 1, 1, SWP_MEM_BOUNDARY},
+    {"acoustic3d_backward_ckpt", 1, 1, SWP_MEM_CKPT},
+    {"acoustic3d_backward_recursive_ckpt", 1, 1, SWP_MEM_CKPT},
+    {"elastic2d_forward", 0, 2, 0},
+    {"elastic2d_backward", 1, 2, SWP_MEM_FULL},
+    {"elastic2d_backward_bs", 1, 2, SWP_MEM_BOUNDARY},
+    {"elastic2d_backward_ckpt", 1, 2, SWP_MEM_CKPT},
+    {"elastic2d_backward_recursive_ckpt", 1, 2, SWP_MEM_CKPT},
+};
+constexpr int kNumEntries = int(sizeof(kEntries) / sizeof(kEntries[0]));
+
+void copy_err(const char *msg, char *err, int cap)
+{
+    if (!err || cap <= 0) return;
+    std::strncpy(err, msg, size_t(cap) - 1);
+    err[cap - 1] = '\0';
+}
+
+template <class F>
+int guarded(char *err, int cap, F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        copy_err(e.what(), err, cap);
+    } catch (...) {
+        copy_err("unknown error in libswp_core", err, cap);
+    }
+    return 1;
+}
+
+const Entry &entry_at(int entry)
+{
+    SWP_CHECK(entry >= 0 && entry < kNumEntries, "entry ", entry, " out of range [0, ", kNumEntries, ")");
+    return kEntries[entry];
+}
+
+void check_problem(const Entry &e, const swp_problem &p)
+{
+    SWP_CHECK(p.B >= 1 && p.nt >= 0, "problem: need B >= 1 and nt >= 0");
+    SWP_CHECK(p.nsrc >= 0 && p.nrec >= 0, "problem: negative point count");
+    SWP_CHECK(p.nsrc == 0 || (p.source && p.sources_loc), "problem: sources missing");
+    SWP_CHECK(p.nrec == 0 || p.receivers_loc, "problem: receivers missing");
+    if (e.kind == 1)
+        SWP_CHECK(p.memory == e.memory, e.name, ": the forward ran with memory mode ", p.memory,
+                  ", this entry serves mode ", e.memory);
+}
+
+}  // namespace
+
+hipStream_t stream() { return t_stream; }
+
+bool eq_available(int eq) { return eq == 0 || (eq == 1 && a3d::available()) || (eq == 2 && e2d::available()); }
+
+}  // namespace swp
+
+using namespace swp;
+
+extern "C" {
+
+SWP_CAPI int swp_core_abi_version(void) { return SWP_ABI_VERSION; }
+
+#define SWP_FIELD(S, f) int64_t(offsetof(S, f))
+static const int64_t kLayout[] = {
+    int64_t(sizeof(swp_problem)),
+    SWP_FIELD(swp_problem, ndim), SWP_FIELD(swp_problem, nz), SWP_FIELD(swp_problem, ny),
+    SWP_FIELD(swp_problem, nx), SWP_FIELD(swp_problem, B), SWP_FIELD(swp_problem, M),
+    SWP_FIELD(swp_problem, nt), SWP_FIELD(swp_problem, abcn), SWP_FIELD(swp_problem, pad),
+    SWP_FIELD(swp_problem, dt), SWP_FIELD(swp_problem, dz), SWP_FIELD(swp_problem, dy),
+    SWP_FIELD(swp_problem, dx), SWP_FIELD(swp_problem, lap_coes), SWP_FIELD(swp_problem, grad_coes),
+    SWP_FIELD(swp_problem, pml), SWP_FIELD(swp_problem, models), SWP_FIELD(swp_problem, model_batched),
+    SWP_FIELD(swp_problem, nsrc), SWP_FIELD(swp_problem, nrec), SWP_FIELD(swp_problem, source),
+    SWP_FIELD(swp_problem, sources_loc), SWP_FIELD(swp_problem, receivers_loc),
+    SWP_FIELD(swp_problem, memory), SWP_FIELD(swp_problem, ckpt_interval), SWP_FIELD(swp_problem, flags),
+    SWP_FIELD(swp_problem, reserved), SWP_FIELD(swp_problem, tape), SWP_FIELD(swp_problem, tape_bytes),
+    SWP_FIELD(swp_problem, workspace), SWP_FIELD(swp_problem, workspace_bytes),
+    int64_t(sizeof(swp_forward_io)), SWP_FIELD(swp_forward_io, record),
+    int64_t(sizeof(swp_backward_io)), SWP_FIELD(swp_backward_io, adjoint_source),
+    SWP_FIELD(swp_backward_io, grad_source), SWP_FIELD(swp_backward_io, grad_models),
+    SWP_FIELD(swp_backward_io, illum_source), SWP_FIELD(swp_backward_io, illum_receiver),
+};
+#undef SWP_FIELD
+
+SWP_CAPI int64_t swp_layout_count(void) { return int64_t(sizeof(kLayout) / sizeof(kLayout[0])); }
+
+SWP_CAPI int64_t swp_layout_probe(int64_t i)
+{
+    if (i < 0 || i >= swp_layout_count()) return -1;
+    return kLayout[i];
+}
+
+SWP_CAPI int swp_entry_count(void) { return kNumEntries; }
+
+SWP_CAPI const char *swp_entry_name(int entry)
+{
+    return (entry >= 0 && entry < kNumEntries) ? kEntries[entry].name : nullptr;
+}
+
+SWP_CAPI int swp_entry_kind(int entry)
+{
+    return (entry >= 0 && entry < kNumEntries) ? kEntries[entry].kind : -1;
+}
+
+SWP_CAPI int swp_entry_available(int entry)
+{
+    if (entry < 0 || entry >= kNumEntries) return -1;
+    return eq_available(kEntries[entry].equation) ? 1 : 0;
+}
+
+SWP_CAPI const char *swp_core_device(void) { return "cuda"; }
+
+SWP_CAPI void swp_set_stream(void *s) { t_stream = static_cast<hipStream_t>(s); }
+
+SWP_CAPI void *swp_get_stream(void) { return static_cast<void *>(t_stream); }
+
+SWP_CAPI int swp_query(int entry, const swp_problem *in, int64_t *workspace_bytes, int64_t *tape_bytes,
+                       char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] {
+        SWP_CHECK(in != nullptr, "swp_query: null problem");
+        const Entry &e = entry_at(entry);
+        check_problem(e, *in);
+        int64_t ws = 0, tp = 0;
+        switch (e.equation) {
+        case 0:
+            ws = a2d::workspace_bytes(*in, e.kind);
+            tp = a2d::tape_bytes(*in);
+            break;
+        case 1:
+            ws = a3d::workspace_bytes(*in, e.kind);
+            tp = a3d::tape_bytes(*in);
+            break;
+        case 2:
+            ws = e2d::workspace_bytes(*in, e.kind);
+            tp = e2d::tape_bytes(*in);
+            break;
+        }
+        if (workspace_bytes) *workspace_bytes = ws;
+        if (tape_bytes) *tape_bytes = tp;
+    });
+}
+
+SWP_CAPI int swp_call(int entry, const swp_problem *in, void *io, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] {
+        SWP_CHECK(in != nullptr && io != nullptr, "swp_call: null problem or io");
+        const Entry &e = entry_at(entry);
+        SWP_CHECK(eq_available(e.equation), e.name, " is not available in this core");
+        check_problem(e, *in);
+        const swp_problem &p = *in;
+        int64_t ws = 0, tp = 0;
+        if (e.equation == 0) { ws = a2d::workspace_bytes(p, e.kind); tp = a2d::tape_bytes(p); }
+        if (e.equation == 1) { ws = a3d::workspace_bytes(p, e.kind); tp = a3d::tape_bytes(p); }
+        if (e.equation == 2) { ws = e2d::workspace_bytes(p, e.kind); tp = e2d::tape_bytes(p); }
+        SWP_CHECK(p.workspace && p.workspace_bytes >= ws, e.name, ": workspace needs ", ws, " bytes, got ",
+                  p.workspace_bytes);
+        SWP_CHECK(p.memory == SWP_MEM_NONE || (p.tape && p.tape_bytes >= tp), e.name, ": tape needs ", tp,
+                  " bytes, got ", p.tape_bytes);
+        timer().reset();
+        if (e.kind == 0) {
+            auto *f = static_cast<swp_forward_io *>(io);
+            SWP_CHECK(p.nrec == 0 || f->record, e.name, ": record buffer missing");
+            if (e.equation == 0) a2d::forward(a2d::make_ctx(p), p, f->record);
+            if (e.equation == 1) a3d::forward(p, f->record);
+            if (e.equation == 2) e2d::forward(p, f->record);
+        } else {
+            auto *b = static_cast<swp_backward_io *>(io);
+            SWP_CHECK(p.nrec == 0 || b->adjoint_source, e.name, ": adjoint source missing");
+            if (e.equation == 0) a2d::backward(a2d::make_ctx(p), p, *b);
+            if (e.equation == 1) a3d::backward(p, *b);
+            if (e.equation == 2) e2d::backward(p, *b);
+        }
+    });
+}
+
+SWP_CAPI void swp_set_timing(int on) { timer().on = on != 0; }
+
+SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launches)
+{
+    char err[8];
+    return guarded(err, int(sizeof(err)), [&] {
+        SWP_CHECK(which >= 0 && which < K_NCLASS, "kernel class out of range");
+        double ms = 0.0;
+        int64_t n = 0;
+        timer().stats(which, &ms, &n);
+        if (ms_total) *ms_total = ms;
+        if (launches) *launches = n;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""GPU parity of the 2-D acoustic path: libswp_core through PropTorch against
+the oracle (same inputs, same memory mode) and the reference's golden vectors.
+
+Bars (north star): records rel-L2 <= 1e-5 and model gradients rel-L2 <= 1e-4
+against the oracle; CPML profiles and index maps bit-exact (checked on the
+host in test_tables.py / test_host.py).  Against the reference's eager golden
+vectors the bar is the reference's own compiled-vs-eager tolerance (1e-3,
+test/test_host_core.py:229): the two differ in the outer-edge treatment, see
+test_oracle_acoustic2d.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import GOLDEN, cosine, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["c1_order4", "het_order8", "het_order6", "het_order2", "het_order10"]
+MODES = {"full": 0, "bs": 1, "ckpt": 0}
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def _memory(name):
+    from sweep_amd.options import BoundarySaving, Ckpt, Full
+    return {"full": Full(), "bs": BoundarySaving(), "ckpt": Ckpt(chunks=7)}[name]
+
+
+def _prop(z, mode, dev, illumination=False):
+    from sweep_amd.equations import Acoustic
+    from sweep_amd.propagator import PropTorch
+    return PropTorch(Acoustic(spatial_order=int(z["order"]), device=dev), shape=tuple(int(s) for s in z["shape"]),
+                     dh=float(z["dh"]), dt=float(z["dt"]), device=dev, abcn=int(z["abcn"]),
+                     memory=_memory(mode), compute_illumination=illumination)
+
+
+def _run(z, mode, dev, illumination=False):
+    solver = _prop(z, mode, dev, illumination)
+    src, rec = np.asarray(z["sources"]), np.asarray(z["receivers"])
+    with torch.no_grad():
+        obs = solver(z["wavelet"], src, rec, models=[torch.tensor(z["vp_true"], device=dev)])
+    vp = torch.tensor(z["vp_init"], device=dev, requires_grad=True)
+    w = torch.tensor(z["wavelet"], device=dev, requires_grad=True)
+    pred = solver(w, src, rec, models=[vp])
+    loss = 0.5 * (pred - obs).pow(2).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    out = dict(obs=obs.cpu().numpy(), pred=pred.detach().cpu().numpy(), grad_vp=vp.grad.cpu().numpy(),
+               grad_wavelet=w.grad.cpu().numpy())
+    if illumination:
+        S, R = solver.collect_illumination()
+        out.update(S=S.cpu().numpy(), R=R.cpu().numpy())
+    return out
+
+
+_oracle_cache = {}
+
+
+def _oracle(name, mode):
+    key = (name, mode)
+    if key in _oracle_cache:
+        return _oracle_cache[key]
+    from oracle.oracle import a2d_from_golden
+    z = np.load(GOLDEN / f"acoustic2d_{name}.npz")
+    P, wav, sl, rl = a2d_from_golden(z)
+    obs = P.forward(P.pad_model(z["vp_true"]), wav, sl, rl)
+    vpi = P.pad_model(z["vp_init"])
+    pred = P.forward(vpi, wav, sl, rl)
+    out = P.gradient(MODES[mode], vpi, wav, sl, rl, pred - obs, illumination=True)
+    res = dict(obs=obs, pred=pred, grad_vp=P.crop(out["grad_vp"]).sum(0),
+               grad_wavelet=out["grad_wavelet"].sum((0, 1)),
+               S=P.crop(out["source_illumination"]).sum(0), R=P.crop(out["receiver_illumination"]).sum(0))
+    _oracle_cache[key] = res
+    return res
+
+
+def _raw(canon):
+    return np.asarray(canon)[..., 0].transpose(0, 2, 1)
+
+
+# Record bar per case.  c1's true-model record (two layers, order 4, receiver
+# on the interface) is rounding-sensitive: the oracle's own source built with
+# FMA contraction moves it by 4.1e-5 (test_oracle_acoustic2d.py::
+# test_fma_contraction_sensitivity_of_c1), and the GPU contracts as the
+# reference's --use_fast_math CUDA core does; its bar is that measured floor.
+RECORD_BAR = {("c1_order4", "obs"): 6e-5}
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("mode", ["full", "bs", "ckpt"])
+def test_matches_oracle(name, mode):
+    dev = _dev()
+    print(f"[acoustic2d] case={name} mode={mode}", flush=True)
+    z = np.load(GOLDEN / f"acoustic2d_{name}.npz")
+    got = _run(z, mode, dev)
+    ref = _oracle(name, mode)
+    errs = dict(obs=rel_l2(_raw(got["obs"]), ref["obs"]), pred=rel_l2(_raw(got["pred"]), ref["pred"]),
+                grad_vp=rel_l2(got["grad_vp"], ref["grad_vp"]),
+                grad_wavelet=rel_l2(got["grad_wavelet"], ref["grad_wavelet"]))
+    print(f"[acoustic2d] {name} {mode} rel-L2 {errs}", flush=True)
+    assert errs["obs"] <= RECORD_BAR.get((name, "obs"), 1e-5)
+    assert errs["pred"] <= RECORD_BAR.get((name, "pred"), 1e-5)
+    assert errs["grad_vp"] <= 1e-4
+    assert errs["grad_wavelet"] <= 1e-4
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_matches_reference_golden(name):
+    dev = _dev()
+    z = np.load(GOLDEN / f"acoustic2d_{name}.npz")
+    got = _run(z, "bs", dev)
+    assert rel_l2(got["obs"], z["obs"]) <= 1e-3
+    assert rel_l2(got["pred"], z["pred"]) <= 1e-3
+    assert cosine(got["grad_vp"], z["grad_vp"]) >= 0.9999
+    assert rel_l2(got["grad_vp"], z["grad_vp"]) <= 1e-3
+    assert rel_l2(got["grad_wavelet"], z["grad_wavelet"]) <= 1e-3
+
+
+def test_memory_modes_agree():
+    dev = _dev()
+    z = np.load(GOLDEN / "acoustic2d_het_order8.npz")
+    full = _run(z, "full", dev)
+    bs = _run(z, "bs", dev)
+    ck = _run(z, "ckpt", dev)
+    assert np.array_equal(full["pred"], bs["pred"]) and np.array_equal(full["pred"], ck["pred"])
+    assert np.array_equal(full["grad_wavelet"], bs["grad_wavelet"])
+    assert np.array_equal(full["grad_vp"], ck["grad_vp"])          # the replay is the forward, bit for bit
+    assert rel_l2(bs["grad_vp"], full["grad_vp"]) <= 1e-4          # the reconstruction is not
+
+
+def test_run_to_run_bit_exact():
+    dev = _dev()
+    z = np.load(GOLDEN / "acoustic2d_het_order8.npz")
+    a = _run(z, "bs", dev)
+    b = _run(z, "bs", dev)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("mode", ["full", "bs"])
+def test_illumination(mode):
+    dev = _dev()
+    z = np.load(GOLDEN / "acoustic2d_het_order6.npz")
+    got = _run(z, mode, dev, illumination=True)
+    ref = _oracle("het_order6", mode)
+    assert rel_l2(got["S"], ref["S"]) <= 1e-4
+    assert rel_l2(got["R"], ref["R"]) <= 1e-4
+    plain = _run(z, mode, dev, illumination=False)
+    assert np.array_equal(plain["grad_vp"], got["grad_vp"])       # illumination never moves the gradient
+
+
+def test_refusals():
+    dev = _dev()
+    z = np.load(GOLDEN / "acoustic2d_het_order2.npz")
+    solver = _prop(z, "bs", dev)
+    src, rec = np.asarray(z["sources"]), np.asarray(z["receivers"])
+    with pytest.raises(TypeError):
+        solver(z["wavelet"], src, rec, models=[torch.tensor(z["vp_init"], device=dev, dtype=torch.float64)])
+    vp = torch.tensor(z["vp_init"], device=dev, requires_grad=True)
+    out = solver(z["wavelet"], src, rec, models=[vp])
+    loss = out.pow(2).sum()
+    loss.backward(retain_graph=True)
+    with pytest.raises(RuntimeError):
+        loss.backward()
