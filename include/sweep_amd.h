@@ -126,10 +126,13 @@ SWP_CAPI int swp_call(int entry, const swp_problem *in, void *io, char *err, int
 SWP_CAPI void swp_set_stream(void *stream);
 SWP_CAPI void *swp_get_stream(void);
 
-/* Wall time of the last call's launches on the device, in ms, measured with
- * HIP events on the call's stream (0 if timing is off).  No reference
- * counterpart: the profiling hook of bench.py. */
-SWP_CAPI void swp_set_timing(int on);
+/* Kernel-class timing with HIP events on the launch stream.  stride > 0 turns
+ * it on (and clears earlier samples), timing one launch in every `stride` of
+ * each class; 0 turns it off.  swp_last_kernel_stats returns the summed
+ * duration (ms) and count of the sampled launches of class `which` (0 forward
+ * step, 1 adjoint step, 2 reconstruction, 3 the rest) since then.  No
+ * reference counterpart: the roofline hook of bench.py. */
+SWP_CAPI void swp_set_timing(int stride);
 SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launches);
 
 #ifdef __cplusplus

@@ -171,8 +171,9 @@ def call(name: str, problem: SwpProblem, io, device=None):
     _raise(rc, err, name)
 
 
-def set_timing(on: bool):
-    load().swp_set_timing(1 if on else 0)
+def set_timing(stride: int):
+    """Time one launch in every ``stride`` per kernel class (0: off)."""
+    load().swp_set_timing(int(stride))
 
 
 def kernel_stats(which: int):

@@ -191,7 +191,6 @@ SWP_CAPI int swp_call(int entry, const swp_problem *in, void *io, char *err, int
                   p.workspace_bytes);
         SWP_CHECK(p.memory == SWP_MEM_NONE || (p.tape && p.tape_bytes >= tp), e.name, ": tape needs ", tp,
                   " bytes, got ", p.tape_bytes);
-        timer().reset();
         if (e.kind == 0) {
             auto *f = static_cast<swp_forward_io *>(io);
             SWP_CHECK(p.nrec == 0 || f->record, e.name, ": record buffer missing");
@@ -208,7 +207,13 @@ SWP_CAPI int swp_call(int entry, const swp_problem *in, void *io, char *err, int
     });
 }
 
-SWP_CAPI void swp_set_timing(int on) { timer().on = on != 0; }
+SWP_CAPI void swp_set_timing(int stride)
+{
+    KTimer &t = timer();
+    t.on = stride > 0;
+    t.stride = stride > 0 ? stride : 1;
+    if (t.on) t.reset();
+}
 
 SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launches)
 {

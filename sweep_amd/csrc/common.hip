@@ -75,19 +75,24 @@ void launch_points(const PointArgs &a, int B)
 // ----------------------------------------------------------------- timer
 KTimer &timer()
 {
-    static thread_local KTimer t;
+    // One per process: torch runs a backward on its own worker thread, so a
+    // thread-local timer would miss the backward's launches.  Calls that
+    // time concurrently from several threads share it (samples interleave).
+    static KTimer t;
     return t;
 }
 
 void KTimer::reset()
 {
     for (auto &m : marks) m.clear();
+    for (auto &s : seen) s = 0;
     next = 0;
 }
 
 int KTimer::begin(int cls)
 {
     if (!on) return -1;
+    if (seen[cls]++ % stride != 0) return -1;
     if (next + 2 > pool.size()) {
         size_t grow = pool.size() < 256 ? 256 : pool.size();
         for (size_t k = 0; k < grow; ++k) {

@@ -63,10 +63,15 @@ struct Carver {
 // bracket the launches of each class with events from a reused pool.
 enum KClass { K_FWD = 0, K_ADJ = 1, K_RECON = 2, K_OTHER = 3, K_NCLASS = 4 };
 
+// Timing samples one launch in every `stride` of a class (an unbiased mean
+// of the launch duration at bounded event count); samples accumulate across
+// calls until swp_set_timing turns timing on again.
 struct KTimer {
     bool on = false;
+    int stride = 1;
     std::vector<hipEvent_t> pool;
     std::vector<std::pair<int, int>> marks[K_NCLASS];   // pool index pairs
+    int64_t seen[K_NCLASS] = {0, 0, 0, 0};
     size_t next = 0;
     void reset();
     int begin(int cls);
