@@ -53,10 +53,34 @@ struct AdjFields {
     float *psixo, *psizo, *zetaxo, *zetazo;
 };
 
+// A rectangle of blocks of the z-marching kernels: 64-column strips
+// [s0, s1) times the rows [z0, z1) cut into segments of tz rows.
+struct Rect {
+    int s0, s1, z0, z1, tz;
+};
+
+// Up to four rectangles enumerated back to back by blockIdx.x.
+struct BlockSet {
+    Rect r[4];
+};
+
+// Tiling of the computed box [M, n-M) into 64-column strips x row
+// segments, split into the interior blocks (no CPML term, no strip cell, no
+// db leak cell reaches them; tall segments) and the band blocks around them
+// (short segments: more waves for the heavier, lower-occupancy band kernels).
+struct ZmPlan {
+    int tz, tz_band, nstrip;
+    BlockSet inner, band;
+    int ninner, nband;
+};
+
 struct Ctx {
     Geo g;
     Coef c;
     Prof pr;
+    bool zm;        // z-marching kernels (fixed orders 2..8)
+    ZmPlan plan;
+    int tz_recon;   // segment height of the reconstruction kernel
 };
 
 Geo make_geo(const swp_problem &p);

@@ -8,8 +8,10 @@
 // adjoint_split.cu:27-190, kernels.cuh:472-574, kernels.cu:3-189); the launch
 // structure, memory layout and time loops are this file's own.
 #include "acoustic2d.h"
+#include "acoustic2d_zm.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace swp {
 namespace a2d {
@@ -472,6 +474,10 @@ static void by_order(int M, K &&k)
 void launch_fwd(const Ctx &x, const StepFields &f, float *u_this, float *strip)
 {
     TimedScope ts(K_FWD);
+    if (x.zm) {
+        launch_fwd_zm(x, x.plan, f, u_this, strip);
+        return;
+    }
     const Geo &g = x.g;
     const dim3 grid = grid_for(g.nx - 2 * g.M, g.nz - 2 * g.M, g.B);
     by_order(g.M, [&](auto ms) {
@@ -484,6 +490,10 @@ void launch_fwd(const Ctx &x, const StepFields &f, float *u_this, float *strip)
 void launch_adj(const Ctx &x, const AdjFields &f, const float *img, float *grad)
 {
     TimedScope ts(K_ADJ);
+    if (x.zm) {
+        launch_adj_zm(x, x.plan, f, img, grad);
+        return;
+    }
     const Geo &g = x.g;
     const dim3 grid = grid_for(g.nx - 2 * g.M, g.nz - 2 * g.M, g.B);
     by_order(g.M, [&](auto ms) {
@@ -789,13 +799,18 @@ void backward(const Ctx &x, const swp_problem &p, const swp_backward_io &io)
         const float *lam = a.l[1];
         if (bxl < g.nx - bxh && bzl < g.nz - bzh) {
             TimedScope ts(K_RECON);
-            by_order(g.M, [&](auto ms) {
-                hipLaunchKernelGGL(recon_kernel<decltype(ms)::value>,
-                                   grid_for(g.nx - bxh - bxl, g.nz - bzh - bzl, g.B), dim3(64, 4), 0,
-                                   stream(), g, x.c, p.models[0], r[0], r[1], r[2], lam, grad, bxl,
-                                   g.nx - bxh, bzl, g.nz - bzh);
-            });
-            SWP_LAUNCHED();
+            if (x.zm) {
+                launch_recon_zm(x, p.models[0], r[0], r[1], r[2], lam, grad, bxl, g.nx - bxh, bzl,
+                                g.nz - bzh, x.tz_recon);
+            } else {
+                by_order(g.M, [&](auto ms) {
+                    hipLaunchKernelGGL(recon_kernel<decltype(ms)::value>,
+                                       grid_for(g.nx - bxh - bxl, g.nz - bzh - bzl, g.B), dim3(64, 4), 0,
+                                       stream(), g, x.c, p.models[0], r[0], r[1], r[2], lam, grad, bxl,
+                                       g.nx - bxh, bzl, g.nz - bzh);
+                });
+                SWP_LAUNCHED();
+            }
         }
         if (n_strip > 0) {
             TimedScope ts(K_OTHER);
@@ -840,6 +855,15 @@ Ctx make_ctx(const swp_problem &p)
     SWP_CHECK(x.g.X1 > x.g.X0 && x.g.Z1 > x.g.Z0, "acoustic2d: empty physical box");
     x.c = make_coef(p, p.lap_coes, p.grad_coes);
     x.pr = Prof{p.pml[0], p.pml[1], p.pml[2], p.pml[3], p.pml[4], p.pml[5]};
+    // SWP_KERNELS=percell keeps the one-thread-per-cell kernels (any order);
+    // SWP_TZ sets the z-segment height of the z-marching kernels.
+    const char *kern = std::getenv("SWP_KERNELS");
+    x.zm = zm_supported(x.g) && !(kern && std::string(kern) == "percell");
+    const char *tz = std::getenv("SWP_TZ");
+    const char *tzr = std::getenv("SWP_TZ_RECON");
+    x.tz_recon = tzr ? std::max(1, std::atoi(tzr)) : 16;
+    const char *tzb = std::getenv("SWP_TZ_BAND");
+    x.plan = make_plan(x.g, tz ? std::max(1, std::atoi(tz)) : 32, tzb ? std::max(1, std::atoi(tzb)) : 8);
     return x;
 }
 
