@@ -47,6 +47,24 @@ void orc_a2d_adj_step(const orc_a2d *g, const float *vp, const float *l_prev, co
                       const float *zetaz, float *psixo, float *psizo, float *zetaxo,
                       float *zetazo, const float *img, float *grad);
 
+/* One 3-D acoustic problem on the runtime grid (nz, ny, nx); x fastest. */
+typedef struct {
+    int nz, ny, nx, B, M, nt;
+    int vp_batched, abcn;
+    int pad[6];           /* z_lo, z_hi, y_lo, y_hi, x_lo, x_hi */
+    float dt, dx, dy, dz;
+    const float *az, *bz, *dbz, *ay, *by, *dby, *ax, *bx, *dbx;
+    const float *lap, *grad, *slap;
+    float slap_c0;
+    const float *sgrad;
+} orc_a3d;
+
+void orc_a3d_forward(const orc_a3d *g, const float *vp, const float *wav, const int *sloc, int nsrc,
+                     const int *rloc, int nrec, float *rec);
+int orc_a3d_gradient(const orc_a3d *g, int mode, const float *vp, const float *wav, const int *sloc, int nsrc,
+                     const int *rloc, int nrec, const float *res, float *rec, float *grad_vp, float *grad_wav,
+                     float *illum_s, float *illum_r);
+
 #ifdef __cplusplus
 }
 #endif

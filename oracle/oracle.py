@@ -63,6 +63,8 @@ def lib(variant="ieee"):
         L.orc_a2d_fwd_step.restype = None
         L.orc_a2d_adj_step.restype = None
         L.orc_set_edge_mode.restype = None
+        L.orc_a3d_forward.restype = None
+        L.orc_a3d_gradient.restype = ctypes.c_int
         _libs[variant] = L
     return _libs[variant]
 
@@ -88,6 +90,7 @@ class A2DProblem:
         from sweep_amd import pml, scalars
         M = order // 2
         self.M, self.nt, self.B = M, nt, B
+        self.abcn, self.dt, self.dh_zyx = abcn, dt, (dh, dh)
         self.variant = variant
         pad = [abcn] * 4
         self.pad = pad
@@ -159,6 +162,101 @@ class A2DProblem:
         if rc != 0:
             raise MemoryError("oracle allocation failed")
         del n
+        out = dict(rec=rec, grad_vp=gv, grad_wavelet=gw)
+        if illumination:
+            out.update(source_illumination=S, receiver_illumination=R)
+        return out
+
+
+class OrcA3D(ctypes.Structure):
+    _fields_ = [
+        ("nz", ctypes.c_int), ("ny", ctypes.c_int), ("nx", ctypes.c_int), ("B", ctypes.c_int),
+        ("M", ctypes.c_int), ("nt", ctypes.c_int), ("vp_batched", ctypes.c_int), ("abcn", ctypes.c_int),
+        ("pad", ctypes.c_int * 6), ("dt", ctypes.c_float), ("dx", ctypes.c_float), ("dy", ctypes.c_float),
+        ("dz", ctypes.c_float),
+        ("az", _F), ("bz", _F), ("dbz", _F), ("ay", _F), ("by", _F), ("dby", _F),
+        ("ax", _F), ("bx", _F), ("dbx", _F),
+        ("lap", _F), ("grad", _F), ("slap", _F), ("slap_c0", ctypes.c_float), ("sgrad", _F),
+    ]
+
+
+class A3DProblem:
+    """The 3-D twin of :class:`A2DProblem`: runtime grid (nz, ny, nx) =
+    physical + pad + 2M per axis, coordinates in (x, y, z) order
+    (``core/geometry.py:48-56``)."""
+
+    def __init__(self, order, shape, dh, dt, nt, abcn=50, max_vel=4500.0, pml_freq=25.0, B=1,
+                 variant="ieee"):
+        from sweep_amd import pml, scalars
+        M = order // 2
+        self.M, self.nt, self.B = M, nt, B
+        dh = tuple(dh) if np.ndim(dh) else (dh, dh, dh)
+        self.abcn, self.dt, self.dh_zyx = abcn, dt, dh
+        self.variant = variant
+        pad = [abcn] * 6
+        self.pad = pad
+        self.nz = shape[0] + pad[0] + pad[1] + 2 * M
+        self.ny = shape[1] + pad[2] + pad[3] + 2 * M
+        self.nx = shape[2] + pad[4] + pad[5] + 2 * M
+        prof = pml.cpml_profiles_r(pad, [M] * 6, dt, list(dh), max_vel, pml_freq, (self.nz, self.ny, self.nx))
+        self.prof = [_f32(p) for p in prof]
+        lap, grad = scalars.laplace_and_gradient_tables(M)
+        self.lap, self.grad = _f32(lap), _f32(grad)
+        sl = scalars.static_laplace_constants(M)
+        sg = scalars.static_gradient_constants(M)
+        self.slap = _f32(sl) if sl is not None else _f32(np.zeros(1))
+        self.sgrad = _f32(sg) if sg is not None else _f32(np.zeros(1))
+        c0 = scalars.static_laplace_c0(M)
+        g = OrcA3D()
+        g.nz, g.ny, g.nx, g.B, g.M, g.nt = self.nz, self.ny, self.nx, B, M, nt
+        g.vp_batched, g.abcn = 1, abcn
+        for k in range(6):
+            g.pad[k] = pad[k]
+        g.dt, g.dz, g.dy, g.dx = dt, dh[0], dh[1], dh[2]
+        (g.az, g.bz, g.dbz, g.ay, g.by, g.dby, g.ax, g.bx, g.dbx) = (_fp(p) for p in self.prof)
+        g.lap, g.grad = _fp(self.lap), _fp(self.grad)
+        g.slap, g.sgrad = _fp(self.slap), _fp(self.sgrad)
+        g.slap_c0 = float(c0) if c0 is not None else 0.0
+        self.g = g
+
+    def pad_model(self, vp):
+        vp = np.asarray(vp, dtype=np.float32)
+        if vp.ndim == 3:
+            vp = np.broadcast_to(vp, (self.B,) + vp.shape)
+        q = [p + self.M for p in self.pad]
+        return _f32(np.pad(vp, ((0, 0), (q[0], q[1]), (q[2], q[3]), (q[4], q[5])), mode="edge"))
+
+    def shift(self, loc):
+        """Physical (x, y, z) int coords (B, n, 3) -> runtime int32."""
+        loc = np.array(loc, dtype=np.int64).copy()
+        loc[..., 0] += self.pad[4] + self.M
+        loc[..., 1] += self.pad[2] + self.M
+        loc[..., 2] += self.pad[0] + self.M
+        return np.ascontiguousarray(loc, dtype=np.int32)
+
+    def crop(self, a):
+        p, M = self.pad, self.M
+        return a[..., p[0] + M:self.nz - p[1] - M, p[2] + M:self.ny - p[3] - M, p[4] + M:self.nx - p[5] - M]
+
+    def forward(self, vp_pad, wav, sloc, rloc):
+        rec = np.zeros((self.B, rloc.shape[1], self.nt), dtype=np.float32)
+        lib(self.variant).orc_a3d_forward(ctypes.byref(self.g), _fp(_f32(vp_pad)), _fp(_f32(wav)), _ip(sloc),
+                                          sloc.shape[1], _ip(rloc), rloc.shape[1], _fp(rec))
+        return rec
+
+    def gradient(self, mode, vp_pad, wav, sloc, rloc, res, illumination=False):
+        B, nt = self.B, self.nt
+        nsrc, nrec = sloc.shape[1], rloc.shape[1]
+        rec = np.zeros((B, nrec, nt), dtype=np.float32)
+        gv = np.zeros((B, self.nz, self.ny, self.nx), dtype=np.float32)
+        gw = np.zeros((B, nsrc, nt), dtype=np.float32)
+        S = np.zeros_like(gv) if illumination else None
+        R = np.zeros_like(gv) if illumination else None
+        rc = lib(self.variant).orc_a3d_gradient(ctypes.byref(self.g), int(mode), _fp(_f32(vp_pad)), _fp(_f32(wav)),
+                                                _ip(sloc), nsrc, _ip(rloc), nrec, _fp(_f32(res)), _fp(rec),
+                                                _fp(gv), _fp(gw), _fp(S), _fp(R))
+        if rc != 0:
+            raise MemoryError("oracle allocation failed")
         out = dict(rec=rec, grad_vp=gv, grad_wavelet=gw)
         if illumination:
             out.update(source_illumination=S, receiver_illumination=R)

@@ -1,0 +1,161 @@
+// ORACLE -- test infrastructure only.
+//
+// A C entry point around the reference's own compiled path (its host core:
+// the reference's CUDA sources built by g++ with its host shims, which is how
+// the reference itself runs impl='c' on a CPU; backend/c/hostcore.py).  It
+// fills the reference's ForwardInputCore / BackwardInputCore
+// (csrc/core/input_core.h) from plain pointers and calls the equation's
+// forward_core / backward_core (full-storage memory mode) directly -- the
+// reference's capi.cu dispatcher is not compiled.  Built by oracle/build_ref.sh
+// into oracle/_ref/ (never committed, never shipped); used by tests to pin
+// the C oracle to the reference's compiled arithmetic.
+#include <cstring>
+#include <exception>
+#include <vector>
+
+#include "core/input_core.h"
+#include "core/outputs.h"
+#include "cuda/equations/acoustic2d/acoustic2d.h"
+#include "cuda/equations/acoustic3d/acoustic3d.h"
+
+namespace {
+
+Buf mk(const void *p, std::vector<int64_t> sizes, int elem = 4)
+{
+    Buf b;
+    b.data_ = const_cast<void *>(p);
+    b.ndim_ = int32_t(sizes.size());
+    int64_t st = 1;
+    for (int d = int(sizes.size()) - 1; d >= 0; --d) {
+        b.sizes_[d] = sizes[d];
+        b.strides_[d] = st;
+        st *= sizes[d];
+    }
+    b.numel_ = st;
+    b.elem_size_ = elem;
+    b.defined_ = true;
+    b.is_cuda_ = false;
+    b.device_ = -1;
+    return b;
+}
+
+BufList list(const std::vector<Buf> &v) { return BufList{v.data(), int64_t(v.size())}; }
+
+void put_err(const char *m, char *err, int cap)
+{
+    if (err && cap > 0) {
+        std::strncpy(err, m, size_t(cap) - 1);
+        err[cap - 1] = 0;
+    }
+}
+
+// grid: (nz, nx) in 2-D, (nz, ny, nx) in 3-D
+template <int NDIM>
+int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zyx,
+        const float *const *pml, const float *vp, const float *lap, const float *grad, const float *src,
+        const int *sloc, int nsrc, const int *rloc, int nrec, const float *res, float *rec_out,
+        float *grad_vp, float *grad_w, char *err, int cap)
+{
+    try {
+        const int nz = grid[0], ny = NDIM == 3 ? grid[1] : 1, nx = grid[NDIM - 1];
+        const int64_t n = int64_t(nz) * ny * nx;
+        std::vector<int64_t> gshape = NDIM == 3 ? std::vector<int64_t>{B, 1, nz, ny, nx}
+                                                : std::vector<int64_t>{B, 1, nz, nx};
+        const int nfwd = NDIM == 3 ? 12 : 9, nadj = NDIM == 3 ? 15 : 11;
+        std::vector<std::vector<float>> store(nfwd + nadj, std::vector<float>(size_t(B * n), 0.f));
+        std::vector<Buf> fw, aw;
+        for (int k = 0; k < nfwd; ++k) fw.push_back(mk(store[k].data(), gshape));
+        for (int k = 0; k < nadj; ++k) aw.push_back(mk(store[nfwd + k].data(), gshape));
+        std::vector<Buf> models{mk(vp, gshape)};
+        std::vector<Buf> pml_b;
+        const int axes[3] = {nz, ny, nx};
+        if (NDIM == 2) {
+            for (int k = 0; k < 3; ++k) pml_b.push_back(mk(pml[k], {nz}));
+            for (int k = 3; k < 6; ++k) pml_b.push_back(mk(pml[k], {nx}));
+        } else {
+            for (int k = 0; k < 9; ++k) pml_b.push_back(mk(pml[k], {axes[k / 3]}));
+        }
+        // spacing in Cartesian order (x[, y], z), propagator/_c.py:1129-1132
+        std::vector<float> spacing;
+        if (NDIM == 2) spacing = {spacing_zyx[1], spacing_zyx[0]};
+        else spacing = {spacing_zyx[2], spacing_zyx[1], spacing_zyx[0]};
+        const int fidx = 0;
+        std::vector<float> u_allt(size_t(nt) * B * n, 0.f);
+        std::vector<int64_t> ashape{nt};
+        for (auto s : gshape) if (&s != &gshape[1]) ashape.push_back(s);
+        ashape = NDIM == 3 ? std::vector<int64_t>{nt, B, nz, ny, nx} : std::vector<int64_t>{nt, B, nz, nx};
+
+        ForwardInputCore f{};
+        f.models = list(models);
+        f.source = mk(src, {B, nsrc, nt});
+        f.lap_coes = mk(lap, {M + 1});
+        f.grad_coes = mk(grad, {M + 1});
+        f.M = M;
+        f.abcn = abcn;
+        f.sources_loc = mk(sloc, {B, nsrc, NDIM});
+        f.receivers_loc = mk(rloc, {B, nrec, NDIM});
+        f.source_field_indices = IntSpan{&fidx, 1};
+        f.receiver_field_indices = IntSpan{&fidx, 1};
+        f.pml_vals = list(pml_b);
+        f.wavefields = list(fw);
+        f.save_all_wavefields = res != nullptr;
+        f.use_boundary_saving = false;
+        f.nt = unsigned(nt);
+        f.dt = dt;
+        f.spacing = FloatSpan{spacing.data(), int64_t(spacing.size())};
+        f.record_out = mk(rec_out, {B, nrec, nt});
+        if (res) f.u_allt_out = mk(u_allt.data(), ashape);
+        if (NDIM == 2) acoustic2d::forward_core(f);
+        else acoustic3d::forward_core(f);
+        if (!res) return 0;
+
+        std::vector<Buf> grads{mk(grad_w, {B, nsrc, nt}), mk(grad_vp, gshape)};
+        BackwardInputCore bi{};
+        bi.u_forward = mk(u_allt.data(), ashape);
+        bi.adjoint_wavefields = list(aw);
+        bi.models = list(models);
+        bi.adjoint_source = mk(res, {B, nrec, nt});
+        bi.forward_source = mk(src, {B, nsrc, nt});
+        bi.lap_coes = f.lap_coes;
+        bi.grad_coes = f.grad_coes;
+        bi.M = M;
+        bi.abcn = abcn;
+        bi.adjoint_sources_loc = f.receivers_loc;
+        bi.forward_sources_loc = f.sources_loc;
+        bi.source_field_indices = f.source_field_indices;
+        bi.receiver_field_indices = f.receiver_field_indices;
+        bi.pml_vals = f.pml_vals;
+        bi.nt = unsigned(nt);
+        bi.dt = dt;
+        bi.spacing = f.spacing;
+        bi.compute_illumination = false;
+        bi.grads_out = list(grads);
+        if (NDIM == 2) acoustic2d::backward_core(bi);
+        else acoustic3d::backward_core(bi);
+        return 0;
+    } catch (const std::exception &e) {
+        put_err(e.what(), err, cap);
+    } catch (...) {
+        put_err("unknown error", err, cap);
+    }
+    return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+__attribute__((visibility("default"))) int ref_acoustic_run(
+    int ndim, const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zyx,
+    const float *const *pml, const float *vp, const float *lap, const float *grad, const float *src,
+    const int *sloc, int nsrc, const int *rloc, int nrec, const float *res, float *rec_out, float *grad_vp,
+    float *grad_w, char *err, int cap)
+{
+    if (ndim == 2)
+        return run<2>(grid, B, M, nt, abcn, dt, spacing_zyx, pml, vp, lap, grad, src, sloc, nsrc, rloc, nrec, res,
+                      rec_out, grad_vp, grad_w, err, cap);
+    return run<3>(grid, B, M, nt, abcn, dt, spacing_zyx, pml, vp, lap, grad, src, sloc, nsrc, rloc, nrec, res,
+                  rec_out, grad_vp, grad_w, err, cap);
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+"""Write golden vectors from the reference's compiled acoustic path.
+
+Run in the build container only, after ``oracle/build_ref.sh`` (the reference
+tree is not present on GPU machines)::
+
+    cd /tmp && python /root/repo/tests/golden/gen_refcore.py
+
+The reference's CPU host core (its own CUDA sources compiled as C++, see
+``oracle/build_ref.sh``) runs each case in full-storage mode through
+``oracle/ref_harness.cpp``: the forward record, then the adjoint driven by a
+seeded residual, giving the model gradient on the runtime grid and the
+wavelet gradient.  Problem assembly (padding, coordinate shift, CPML
+profiles, FD tables) is the oracle's, whose profiles and tables the table
+tests pin bit-exactly against the reference's own builders.  Only inputs and
+outputs are stored, as ``.npz`` arrays.
+"""
+from __future__ import annotations
+
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+
+OUT = Path(__file__).resolve().parent
+ROOT = OUT.parents[1]
+
+
+def ricker(t, f):
+    a = (np.pi * f * t) ** 2
+    return ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+
+
+def smooth_model(rng, shape, lo, hi):
+    from scipy.ndimage import gaussian_filter
+    v = gaussian_filter(rng.standard_normal(shape), sigma=2.0)
+    v = (v - v.min()) / max(v.max() - v.min(), 1e-12)
+    z = np.linspace(0.0, 1.0, shape[0]).reshape((-1,) + (1,) * (len(shape) - 1))
+    return (lo + (hi - lo) * (0.6 * z + 0.4 * v)).astype(np.float32)
+
+
+CASES_2D = [
+    # name, order, shape, nsrc, B, nt
+    ("o2", 2, (32, 44), 1, 2, 300),
+    ("o4", 4, (32, 44), 1, 2, 300),
+    ("o6", 6, (32, 44), 2, 2, 300),
+    ("o8", 8, (32, 44), 1, 3, 300),
+]
+CASES_3D = [
+    ("o2", 2, (14, 12, 16), 1, 2, 150),
+    ("o4", 4, (14, 12, 16), 1, 2, 150),
+    ("o8", 8, (14, 12, 16), 1, 2, 150),
+]
+
+
+def main():
+    if os.getcwd().startswith("/root/reference"):
+        raise SystemExit("run from outside /root/reference")
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle, refcore
+    if not refcore.available():
+        raise SystemExit("oracle/_ref is not built: run oracle/build_ref.sh first")
+    dh, dt, abcn, f0 = 10.0, 1e-3, 10, 15.0
+    for ndim, cases in ((2, CASES_2D), (3, CASES_3D)):
+        for name, order, shape, nsrc, B, nt in cases:
+            rng = np.random.default_rng(1000 * ndim + order)
+            ab = abcn if ndim == 2 else 8
+            if ndim == 2:
+                P = oracle.A2DProblem(order, shape, dh, dt, nt, abcn=ab, B=B)
+            else:
+                P = oracle.A3DProblem(order, shape, dh, dt, nt, abcn=ab, B=B)
+            vp = smooth_model(rng, shape, 1800.0, 2600.0)
+            wav = ricker(np.arange(nt) * dt - 1.2 / f0, f0)
+            nx = shape[-1]
+            src = np.zeros((B, nsrc, ndim), np.int64)
+            for b in range(B):
+                for s in range(nsrc):
+                    src[b, s, 0] = int(rng.integers(2, nx - 2))
+                    if ndim == 3:
+                        src[b, s, 1] = int(rng.integers(2, shape[1] - 2))
+                    src[b, s, -1] = int(rng.integers(1, 4))
+            nrec = 7
+            rec = np.zeros((B, nrec, ndim), np.int64)
+            rec[..., 0] = np.linspace(0, nx - 1, nrec).astype(np.int64)
+            if ndim == 3:
+                rec[..., 1] = shape[1] // 2
+            rec[..., -1] = 2
+            wavB = np.broadcast_to(wav, (B, nsrc, nt)).copy()
+            res = rng.standard_normal((B, nrec, nt)).astype(np.float32)
+            out = refcore.run(P, P.pad_model(vp), wavB, P.shift(src), P.shift(rec), res)
+            path = OUT / f"refcore{ndim}d_{name}.npz"
+            np.savez_compressed(path, order=order, shape=np.array(shape), dh=dh, dt=dt, nt=nt, abcn=ab, B=B,
+                                vp=vp, wavelet=wav, sources=src, receivers=rec, residual=res,
+                                rec=out["rec"], grad_vp=out["grad_vp"], grad_wavelet=out["grad_wavelet"])
+            print(path.name, os.path.getsize(path))
+
+
+if __name__ == "__main__":
+    main()
