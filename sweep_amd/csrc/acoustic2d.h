@@ -84,6 +84,9 @@ struct Ctx {
 };
 
 Geo make_geo(const swp_problem &p);
+// Coefficient block from the caller's (M+1) tables plus the fixed-order
+// constants; shared with the 3-D path.
+Coef make_coef(const swp_problem &p, const float *lap_host, const float *grad_host);
 Ctx make_ctx(const swp_problem &p);
 int64_t workspace_bytes(const swp_problem &p, int kind);
 int64_t tape_bytes(const swp_problem &p);

@@ -19,6 +19,7 @@
 // reference's expression trees (csrc/cuda/equations/acoustic2d/
 // forward_split.cu:39-168, adjoint_split.cu:27-190, kernels.cuh:472-574).
 #include "acoustic2d_zm.h"
+#include "fd.h"
 
 #include <algorithm>
 
@@ -49,36 +50,8 @@ __device__ __forceinline__ void block_coords(const BlockSet &bs, int &strip, int
     zb = ze = 0;
 }
 
-// Forward second derivative over taps t(-M..M) centred on t(0): order 4 in
-// summed form, the others in difference form (laplace.cuh:68-387).
-template <int M, class T>
-__device__ __forceinline__ float lap_taps(const Coef &c, T t, float inv_h2)
-{
-    const float u0 = t(0);
-    if constexpr (M == 1) {
-        return ((t(-1) - u0) + (t(1) - u0)) * inv_h2;
-    } else if constexpr (M == 2) {
-        return (c.slap[0] * (t(-1) + t(1)) + c.slap[1] * (t(-2) + t(2)) - c.slap_c0 * u0) * inv_h2;
-    } else {
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 1; k <= M; ++k) acc += c.slap[k - 1] * ((t(-k) - u0) + (t(k) - u0));
-        return acc * inv_h2;
-    }
-}
-
-template <int M, class T>
-__device__ __forceinline__ float d1_taps(const Coef &c, T t, float inv_h)
-{
-    if constexpr (M == 1) {
-        return (t(1) - t(-1)) * (0.5f * inv_h);
-    } else {
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 1; k <= M; ++k) acc += c.sgrad[k - 1] * (t(k) - t(-k));
-        return acc * inv_h;
-    }
-}
+using fd::d1_taps;
+using fd::lap_taps;
 
 // The lane's halo duty: lanes 0..M-1 load the M columns left of the strip,
 // lanes M..2M-1 the M columns right of it (clamped in-grid; an out-of-grid
