@@ -1,6 +1,7 @@
-"""Benchmark: 2-D acoustic FWI gradients (forward + adjoint, boundary saving).
+"""Benchmark: acoustic FWI gradients (forward + adjoint, boundary saving).
 
-Workload (one "step" = one gradient over the whole shot batch):
+Default workload, ``acoustic2d`` (one "step" = one gradient over the whole
+shot batch):
   grid 384 x 1152 at dh = 8 m, dt = 0.8 ms, nt = 3000, 8th order in space,
   CPML 50 cells, seeded sinuous-layer vp (1500-4500 m/s, +-5% smoothed noise),
   32 surface shots (Ricker 8 Hz, delay 1.5/f) with uneven receiver spreads
@@ -8,7 +9,14 @@ Workload (one "step" = one gradient over the whole shot batch):
   loss 0.5 ||mask (pred - obs)||^2 against the record of the true model,
   gradient w.r.t. the smoothed starting model by boundary saving.
 
-Prints ONE JSON line: shot-gradients/s (value = 32 x steps / time), the
+``--workload acoustic3d``:
+  grid (nz, ny, nx) = 256 x 512 x 512 at dh = 10 m, dt = 1 ms, nt = 2000,
+  8th order, CPML 50 cells, seeded dipping-layer vp (1500-4000 m/s), 8 shots
+  each recorded by a 64 x 64 surface receiver patch; one step = the 8 shot
+  gradients, computed ``--shots-per-launch`` shots at a time (the boundary
+  strips of one shot take 42 GB at nt = 2000).
+
+Prints ONE JSON line: shot-gradients/s (value = shots x steps / time), the
 roofline of the dominant kernel (algorithmic bytes per launch over the mean
 launch duration, measured with HIP events on the launch stream during the
 timed steps), and the oracle's CPU time for a bounded sample of the same
@@ -30,83 +38,122 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 
-WORKLOAD = dict(name="acoustic2d_384x1152_nt3000_32shots_bs", shape=(384, 1152), dh=8.0, dt=8e-4,
-                nt=3000, order=8, abcn=50, nshots=32, f0=8.0, seed=2)
+WORKLOADS = {
+    "acoustic2d": dict(name="acoustic2d_384x1152_nt3000_32shots_bs", shape=(384, 1152), dh=8.0, dt=8e-4,
+                       nt=3000, order=8, abcn=50, nshots=32, f0=8.0, seed=2, per_launch=32),
+    "acoustic3d": dict(name="acoustic3d_256x512x512_nt2000_8shots_bs", shape=(256, 512, 512), dh=10.0,
+                       dt=1e-3, nt=2000, order=8, abcn=50, nshots=8, f0=8.0, seed=4, per_launch=2),
+}
+
+KERNEL_NAMES = {0: "fwd (forward step + strip save)", 1: "adj (exact adjoint step)",
+                2: "recon (BS reconstruction + fused imaging)"}
 
 
-def make_problem(dev, nt=None, nshots=None):
-    import torch
+# ----------------------------------------------------------------- problems
+def make_problem(W, dev, nt):
+    """(solver, vp_true, vp_init, sources, receivers, mask or None, wavelet)."""
     from sweep_amd import synthetic
-    from sweep_amd.equations import Acoustic
+    from sweep_amd.equations import Acoustic, Acoustic3D
     from sweep_amd.options import BoundarySaving
     from sweep_amd.propagator import PropTorch
-    W = WORKLOAD
-    nt = nt or W["nt"]
-    nshots = nshots or W["nshots"]
-    vp_true = synthetic.layered_2d(W["shape"], seed=W["seed"])
-    vp_init = synthetic.smoothed(vp_true)
-    src, rec, mask = synthetic.surface_acquisition(W["shape"], nshots, seed=W["seed"])
     t = np.arange(nt) * W["dt"]
     wav = synthetic.ricker(t - 1.5 / W["f0"], W["f0"])
-    solver = PropTorch(Acoustic(spatial_order=W["order"], device=dev), shape=W["shape"], dh=W["dh"],
-                       dt=W["dt"], device=dev, abcn=W["abcn"], memory=BoundarySaving())
-    mask_t = torch.tensor(mask, device=dev)[:, None, :, None]
-    return solver, vp_true, vp_init, src, rec, mask_t, wav
+    if len(W["shape"]) == 2:
+        vp_true = synthetic.layered_2d(W["shape"], seed=W["seed"])
+        src, rec, mask = synthetic.surface_acquisition(W["shape"], W["nshots"], seed=W["seed"])
+        eq = Acoustic(spatial_order=W["order"], device=dev)
+    else:
+        vp_true = synthetic.dipping_3d(W["shape"], seed=W["seed"])
+        src, rec = synthetic.patch_acquisition_3d(W["shape"], W["nshots"])
+        mask = None
+        eq = Acoustic3D(spatial_order=W["order"], device=dev)
+    vp_init = synthetic.smoothed(vp_true)
+    solver = PropTorch(eq, shape=W["shape"], dh=W["dh"], dt=W["dt"], device=dev, abcn=W["abcn"],
+                       memory=BoundarySaving())
+    return solver, vp_true, vp_init, src, rec, mask, wav
 
 
-def algorithmic_bytes(B, nz, nx, M, pad, W):
+def algorithmic_bytes(B, grid, M, pad, W):
     """Bytes per launch of the three stencil kernels under the counting rule:
     4 B per distinct full-grid float read or written per cell, perfect halo
     reuse; +16 B per cell per axis inside that axis' CPML compute band (psi,
     zeta read, next psi, zeta written); +4 B per saved strip cell; +12 B per
-    imaged cell (lambda read, gradient read + write)."""
-    iz = np.arange(M, nz - M)[:, None]
-    ix = np.arange(M, nx - M)[None, :]
-    cells = iz.size * ix.size
-    cx = ((ix < pad + 2 * M) | (ix >= nx - pad - 2 * M)).sum() * iz.size
-    cz = ((iz < pad + 2 * M) | (iz >= nz - pad - 2 * M)).sum() * ix.size
-    nxp, nzp = nx - 2 * pad - 2 * M, nz - 2 * pad - 2 * M
-    strip = 2 * W * nxp + 2 * W * nzp
-    fwd = B * (16 * cells + 16 * (cx + cz) + 4 * strip)
-    adj = B * (16 * cells + 16 * (cx + cz))
+    imaged cell (lambda read, gradient read + write).  ``grid`` is the
+    runtime grid, (nz, nx) or (nz, ny, nx)."""
+    ndim = len(grid)
+    comp = [n - 2 * M for n in grid]
+    cells = int(np.prod(comp))
+    phys = [n - 2 * pad - 2 * M for n in grid]
+
+    def band_count(n, lo_w, hi_w):
+        p = np.arange(M, n - M)
+        return int(((p < lo_w) | (p >= n - hi_w)).sum())
+
+    # per-axis band cells: the 2-D forward/adjoint ring is pad + 2M wide;
+    # the 3-D forward band is pad + M + 1, the 3-D adjoint band pad + 2M + 1
+    if ndim == 2:
+        fb = [band_count(n, pad + 2 * M, pad + 2 * M) for n in grid]
+        ab = fb
+    else:
+        fb = [band_count(n, pad + M + 1, pad + M + 1) for n in grid]
+        ab = [band_count(n, pad + 2 * M + 1, pad + 2 * M + 1) for n in grid]
+    fband = sum(fb[a] * cells // comp[a] for a in range(ndim))
+    aband = sum(ab[a] * cells // comp[a] for a in range(ndim))
+    if ndim == 2:
+        strip = 2 * W * phys[1] + 2 * W * phys[0]
+    else:
+        nzp, nyp, nxp = phys
+        strip = 2 * W * (nyp * nxp + nzp * nxp + nzp * nyp)
     lo = pad + 2 * M + 1
-    rcells = (nz - 2 * lo) * (nx - 2 * lo)
-    img = (nzp - 2 * W) * (nxp - 2 * W)
-    recon = B * (16 * rcells + 12 * img)
-    return {0: fwd, 1: adj, 2: recon}
+    rcells = int(np.prod([n - 2 * lo for n in grid]))
+    img = int(np.prod([p - 2 * W for p in phys]))
+    return {0: B * (16 * cells + 16 * fband + 4 * strip), 1: B * (16 * cells + 16 * aband),
+            2: B * (16 * rcells + 12 * img)}
 
 
-def cpu_baseline(sample_nt=1000):
+# ----------------------------------------------------------------- CPU leg
+def cpu_baseline(W, sample_nt):
     """The oracle (C, OpenMP) on one shot of the same grid for sample_nt steps
-    of forward + boundary-saving backward, extrapolated to the full nt."""
-    from oracle.oracle import A2DProblem
+    of forward + boundary-saving backward, extrapolated to the full job."""
+    from oracle import oracle
     from sweep_amd import synthetic
-    W = WORKLOAD
-    P = A2DProblem(W["order"], W["shape"], W["dh"], W["dt"], sample_nt, abcn=W["abcn"], B=1)
-    vp = P.pad_model(synthetic.smoothed(synthetic.layered_2d(W["shape"], seed=W["seed"])))
     t = np.arange(sample_nt) * W["dt"]
     wav = np.ascontiguousarray(synthetic.ricker(t - 1.5 / W["f0"], W["f0"])[None, None, :])
-    src, rec, _ = synthetic.surface_acquisition(W["shape"], 1, seed=W["seed"], nrec_range=(300, 300))
+    if len(W["shape"]) == 2:
+        P = oracle.A2DProblem(W["order"], W["shape"], W["dh"], W["dt"], sample_nt, abcn=W["abcn"], B=1)
+        vp = P.pad_model(synthetic.smoothed(synthetic.layered_2d(W["shape"], seed=W["seed"])))
+        src, rec, _ = synthetic.surface_acquisition(W["shape"], 1, seed=W["seed"], nrec_range=(300, 300))
+        grid = f"{P.nz}x{P.nx}"
+    else:
+        P = oracle.A3DProblem(W["order"], W["shape"], W["dh"], W["dt"], sample_nt, abcn=W["abcn"], B=1)
+        vp = P.pad_model(synthetic.smoothed(synthetic.dipping_3d(W["shape"], seed=W["seed"])))
+        src, rec = synthetic.patch_acquisition_3d(W["shape"], 1)
+        grid = f"{P.nz}x{P.ny}x{P.nx}"
     sl, rl = P.shift(src[:, None, :]), P.shift(rec)
     res = np.random.default_rng(0).standard_normal((1, rl.shape[1], sample_nt)).astype(np.float32)
     t0 = time.perf_counter()
     P.gradient(1, vp, wav, sl, rl, res)
     dt = time.perf_counter() - t0
-    per_grad = dt * W["nt"] / sample_nt
+    scale = W["nt"] / sample_nt
+    per_grad = dt * scale
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": 1.0 / per_grad, "unit": "shot-gradients/s", "cores": cores, "kind": "port",
             "sample": f"1 shot, {sample_nt} of {W['nt']} steps of forward + boundary-saving backward "
-                      f"on the full 492x1260 runtime grid ({dt:.1f} s), extrapolated x{W['nt'] // sample_nt}"}
+                      f"on the full {grid} runtime grid ({dt:.1f} s), extrapolated x{scale:g}"}
 
 
+# ----------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="acoustic2d")
     ap.add_argument("--timing-stride", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-nt", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None, help="override nt (profiling runs only)")
+    ap.add_argument("--shots-per-launch", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -114,20 +161,30 @@ def main():
         sys.exit(2)
     import torch
     from sweep_amd import _core
+    W = WORKLOADS[args.workload]
+    nt = args.nt or W["nt"]
+    per_launch = args.shots_per_launch or W["per_launch"]
     dev = torch.device("cuda:0")
-    solver, vp_true, vp_init, src, rec, mask, wav = make_problem(dev, nt=args.nt)
-    W = WORKLOAD
+    solver, vp_true, vp_init, src, rec, mask, wav = make_problem(W, dev, nt)
     nshots = src.shape[0]
-    print(f"[bench] {W['name']}: building obs", file=sys.stderr, flush=True)
+    batches = [(b, min(b + per_launch, nshots)) for b in range(0, nshots, per_launch)]
+    print(f"[bench] {W['name']} nt={nt}: building obs", file=sys.stderr, flush=True)
+    vt = torch.tensor(vp_true, device=dev)
     with torch.no_grad():
-        obs = solver(wav, src, rec, models=[torch.tensor(vp_true, device=dev)])
+        obs = [solver(wav, src[a:b], rec[a:b], models=[vt]) for a, b in batches]
+    del vt
+    masks = [None if mask is None else torch.tensor(mask[a:b], device=dev)[:, None, :, None] for a, b in batches]
     vp0 = torch.tensor(vp_init, device=dev)
 
     def step():
         vp = vp0.clone().requires_grad_(True)
-        pred = solver(wav, src, rec, models=[vp])
-        loss = 0.5 * ((pred - obs) * mask).pow(2).sum()
-        loss.backward()
+        for k, (a, b) in enumerate(batches):
+            pred = solver(wav, src[a:b], rec[a:b], models=[vp])
+            r = pred - obs[k]
+            if masks[k] is not None:
+                r = r * masks[k]
+            loss = 0.5 * r.pow(2).sum()
+            loss.backward()
         return vp.grad
 
     for k in range(args.warmup):
@@ -147,9 +204,8 @@ def main():
     assert torch.isfinite(g).all(), "non-finite gradient"
 
     P = solver._last_call.problem
-    abytes = algorithmic_bytes(nshots, P.nz, P.nx, P.M, W["abcn"], P.M + 1)
-    names = {0: "fwd_kernel (forward step + strip save)", 1: "adj_kernel (exact adjoint step)",
-             2: "recon_kernel (BS reconstruction + fused imaging)"}
+    grid = (P.nz, P.nx) if P.ndim == 2 else (P.nz, P.ny, P.nx)
+    abytes = algorithmic_bytes(P.B, grid, P.M, W["abcn"], P.M + 1)
     per = {}
     for c in (0, 1, 2):
         ms, n = stats[c]
@@ -158,7 +214,6 @@ def main():
                           gbs=abytes[c] / (ms / n * 1e-3) / 1e9)
     dom = max(per, key=lambda c: per[c]["total_est_ms"])
     ach = per[dom]["gbs"]
-    nt = P.nt
     value = nshots * args.steps / elapsed
     out = {
         "metric": "shot-gradients/s fwd+adj", "value": value, "unit": "shot-gradients/s",
@@ -166,19 +221,20 @@ def main():
         "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": W["name"] if args.nt is None else f"{W['name']}_nt{nt}",
-                   "grid": list(W["shape"]), "runtime_grid": [P.nz, P.nx], "nt": nt, "shots": nshots,
-                   "order": W["order"], "memory": "boundary_saving"},
-        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
+                   "grid": list(W["shape"]), "runtime_grid": list(grid), "nt": nt, "shots": nshots,
+                   "shots_per_launch": per_launch, "order": W["order"], "memory": "boundary_saving"},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": int(abytes[dom]),
                      "avg_launch_ms": per[dom]["avg_ms"],
-                     "kernels": {names[c]: {"avg_launch_ms": per[c]["avg_ms"], "GB/s": per[c]["gbs"]}
-                                 for c in per}},
+                     "kernels": {KERNEL_NAMES[c]: {"avg_launch_ms": per[c]["avg_ms"], "GB/s": per[c]["gbs"],
+                                                   "frac": per[c]["gbs"] / HBM_PEAK_GBS} for c in per}},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline()
+        sample = args.cpu_sample_nt or (1000 if len(W["shape"]) == 2 else 4)
+        out["cpu_baseline"] = cpu_baseline(W, sample)
     print(json.dumps(out), flush=True)
 
 

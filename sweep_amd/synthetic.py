@@ -79,6 +79,29 @@ def surface_acquisition(shape, nshots, seed=0, nrec_range=(150, 500), depth=2):
     return src, rec, mask
 
 
+def patch_acquisition_3d(shape, nshots, patch=64, depth=2):
+    """``(sources (nshots, 3), receivers (nshots, patch*patch, 3))`` in
+    physical (x, y, z) cells: sources on a regular surface grid, each shot
+    recorded by a patch x patch receiver square centred on it (clipped to the
+    grid)."""
+    nz, ny, nx = shape
+    ncol = int(np.ceil(np.sqrt(nshots * nx / ny)))
+    nrow = int(np.ceil(nshots / ncol))
+    xs = np.linspace(0.2 * nx, 0.8 * nx, ncol).round().astype(np.int64)
+    ys = np.linspace(0.2 * ny, 0.8 * ny, max(nrow, 1)).round().astype(np.int64)
+    src = np.array([[xs[k % ncol], ys[k // ncol], depth] for k in range(nshots)], dtype=np.int64)
+    off = np.arange(patch) - patch // 2
+    rec = np.zeros((nshots, patch * patch, 3), dtype=np.int64)
+    for s in range(nshots):
+        rx = np.clip(src[s, 0] + off, 0, nx - 1)
+        ry = np.clip(src[s, 1] + off, 0, ny - 1)
+        gy, gx = np.meshgrid(ry, rx, indexing="ij")
+        rec[s, :, 0] = gx.ravel()
+        rec[s, :, 1] = gy.ravel()
+        rec[s, :, 2] = depth
+    return src, rec
+
+
 def dipping_3d(shape, seed=0, vmin=1500.0, vmax=4000.0, nlayers=6):
     rng = np.random.default_rng(seed)
     nz, ny, nx = shape
