@@ -2,7 +2,7 @@
 # ORACLE -- test infrastructure only.
 #
 # Builds oracle/_ref/libref_acoustic.so: the reference's own acoustic 2-D/3-D
-# translation units, compiled from where they lie under /root/reference with
+# and elastic 2-D translation units, compiled from where they lie under /root/reference with
 # g++ as C++ (the way the reference builds its CPU "host core",
 # backend/c/hostcore.py:64-82: its CUDA stand-in headers under
 # csrc/host/include first on the include path, cuda_runtime.h forced in),
@@ -28,6 +28,7 @@ SRCS=(
   cuda/equations/acoustic3d/forward.cu cuda/equations/acoustic3d/forward_split.cu
   cuda/equations/acoustic3d/backward.cu cuda/equations/acoustic3d/adjoint_split.cu
   cuda/equations/acoustic3d/kernels.cu
+  cuda/equations/elastic2d/forward.cu cuda/equations/elastic2d/backward.cu cuda/equations/elastic2d/kernels.cu
   cuda/common/common.cu cuda/common/boundarysaver.cu cuda/common/derived_models.cu
   cuda/common/layout.cu cuda/common/boundary/session.cu
 )

@@ -65,6 +65,27 @@ int orc_a3d_gradient(const orc_a3d *g, int mode, const float *vp, const float *w
                      const int *rloc, int nrec, const float *res, float *rec, float *grad_vp, float *grad_wav,
                      float *illum_s, float *illum_r);
 
+/* One 2-D isotropic elastic problem (velocity-stress, cpmls) on the runtime
+ * grid (nz, nx); fields 0 vx, 1 vz, 2 sxx, 3 szz, 4 sxz. */
+typedef struct {
+    int nz, nx, B, M, nt;
+    int vp_batched;       /* 1: models are (B, nz, nx); 0: shared */
+    int pad[4];           /* z_lo, z_hi, x_lo, x_hi */
+    float dt, dx, dz;
+    const float *az, *bz, *azh, *bzh, *ax, *bx, *axh, *bxh;
+    const float *coef;    /* staggered weights c_1..c_M (index 0..M-1), used for M > 4 */
+    int nsf, nrf;
+    const int *src_fields, *rec_fields;
+} orc_e2d;
+
+/* rec (nrf, B, nrec, nt) */
+void orc_e2d_forward(const orc_e2d *g, const float *vp, const float *vs, const float *rho, const float *wav,
+                     const int *sloc, int nsrc, const int *rloc, int nrec, float *rec);
+/* full-storage gradient for the residual res (nrf, B, nrec, nt); gradients (B, nz, nx) accumulate */
+int orc_e2d_gradient(const orc_e2d *g, const float *vp, const float *vs, const float *rho, const float *wav,
+                     const int *sloc, int nsrc, const int *rloc, int nrec, const float *res, float *rec,
+                     float *gvp, float *gvs, float *grho);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,8 @@ def lib(variant="ieee"):
         L.orc_set_edge_mode.restype = None
         L.orc_a3d_forward.restype = None
         L.orc_a3d_gradient.restype = ctypes.c_int
+        L.orc_e2d_forward.restype = None
+        L.orc_e2d_gradient.restype = ctypes.c_int
         _libs[variant] = L
     return _libs[variant]
 
@@ -261,6 +263,90 @@ class A3DProblem:
         if illumination:
             out.update(source_illumination=S, receiver_illumination=R)
         return out
+
+
+class OrcE2D(ctypes.Structure):
+    _fields_ = [
+        ("nz", ctypes.c_int), ("nx", ctypes.c_int), ("B", ctypes.c_int), ("M", ctypes.c_int),
+        ("nt", ctypes.c_int), ("vp_batched", ctypes.c_int), ("pad", ctypes.c_int * 4),
+        ("dt", ctypes.c_float), ("dx", ctypes.c_float), ("dz", ctypes.c_float),
+        ("az", _F), ("bz", _F), ("azh", _F), ("bzh", _F), ("ax", _F), ("bx", _F), ("axh", _F), ("bxh", _F),
+        ("coef", _F), ("nsf", ctypes.c_int), ("nrf", ctypes.c_int), ("src_fields", _I), ("rec_fields", _I),
+    ]
+
+
+FIELDS = {"vx": 0, "vz": 1, "sxx": 2, "szz": 3, "sxz": 4}
+
+
+class E2DProblem:
+    """Everything the elastic oracle needs: runtime grid (nz, nx), cpmls
+    profiles [az, bz, azh, bzh, ax, bx, axh, bxh] with fd_pad = M, the
+    staggered weights, and the source / receiver field lists."""
+
+    def __init__(self, order, shape, dh, dt, nt, abcn=50, max_vel=4500.0, pml_freq=25.0, B=1,
+                 source_fields=("vz",), receiver_fields=("vx", "vz"), variant="ieee"):
+        from sweep_amd import pml, scalars
+        M = order // 2
+        self.M, self.nt, self.B = M, nt, B
+        self.abcn, self.dt, self.dh_zyx = abcn, dt, (dh, dh)
+        self.variant = variant
+        pad = [abcn] * 4
+        self.pad = pad
+        self.nz = shape[0] + pad[0] + pad[1] + 2 * M
+        self.nx = shape[1] + pad[2] + pad[3] + 2 * M
+        prof = pml.cpml_profiles_s(pad, [M] * 4, dt, [dh, dh], max_vel, pml_freq, (self.nz, self.nx))
+        self.prof = [_f32(p) for p in prof]
+        coef = np.zeros(M + 1, dtype=np.float32)
+        coef[:M] = scalars.staggered_grid_coes(M)
+        self.grad = coef
+        self.sf = np.array([FIELDS[f] for f in source_fields], dtype=np.int32)
+        self.rf = np.array([FIELDS[f] for f in receiver_fields], dtype=np.int32)
+        g = OrcE2D()
+        g.nz, g.nx, g.B, g.M, g.nt, g.vp_batched = self.nz, self.nx, B, M, nt, 1
+        for k in range(4):
+            g.pad[k] = pad[k]
+        g.dt, g.dx, g.dz = dt, dh, dh
+        (g.az, g.bz, g.azh, g.bzh, g.ax, g.bx, g.axh, g.bxh) = (_fp(p) for p in self.prof)
+        g.coef = _fp(self.grad)
+        g.nsf, g.nrf = len(self.sf), len(self.rf)
+        g.src_fields, g.rec_fields = _ip(self.sf), _ip(self.rf)
+        self.g = g
+
+    def pad_model(self, m):
+        m = np.asarray(m, dtype=np.float32)
+        if m.ndim == 2:
+            m = np.broadcast_to(m, (self.B,) + m.shape)
+        p = [q + self.M for q in self.pad]
+        return _f32(np.pad(m, ((0, 0), (p[0], p[1]), (p[2], p[3])), mode="edge"))
+
+    def shift(self, loc):
+        loc = np.array(loc, dtype=np.int64).copy()
+        loc[..., 0] += self.pad[2] + self.M
+        loc[..., 1] += self.pad[0] + self.M
+        return np.ascontiguousarray(loc, dtype=np.int32)
+
+    def crop(self, a):
+        p, M = self.pad, self.M
+        return a[..., p[0] + M:self.nz - p[1] - M, p[2] + M:self.nx - p[3] - M]
+
+    def forward(self, vp, vs, rho, wav, sloc, rloc):
+        rec = np.zeros((len(self.rf), self.B, rloc.shape[1], self.nt), dtype=np.float32)
+        lib(self.variant).orc_e2d_forward(ctypes.byref(self.g), _fp(_f32(vp)), _fp(_f32(vs)), _fp(_f32(rho)),
+                                          _fp(_f32(wav)), _ip(sloc), sloc.shape[1], _ip(rloc), rloc.shape[1],
+                                          _fp(rec))
+        return rec
+
+    def gradient(self, vp, vs, rho, wav, sloc, rloc, res):
+        B, nt = self.B, self.nt
+        rec = np.zeros((len(self.rf), B, rloc.shape[1], nt), dtype=np.float32)
+        g = [np.zeros((B, self.nz, self.nx), dtype=np.float32) for _ in range(3)]
+        rc = lib(self.variant).orc_e2d_gradient(ctypes.byref(self.g), _fp(_f32(vp)), _fp(_f32(vs)),
+                                                _fp(_f32(rho)), _fp(_f32(wav)), _ip(sloc), sloc.shape[1],
+                                                _ip(rloc), rloc.shape[1], _fp(_f32(res)), _fp(rec), _fp(g[0]),
+                                                _fp(g[1]), _fp(g[2]))
+        if rc != 0:
+            raise MemoryError("oracle allocation failed")
+        return dict(rec=rec, grad_vp=g[0], grad_vs=g[1], grad_rho=g[2])
 
 
 def a2d_from_golden(z, variant="ieee"):

@@ -17,6 +17,7 @@
 #include "core/outputs.h"
 #include "cuda/equations/acoustic2d/acoustic2d.h"
 #include "cuda/equations/acoustic3d/acoustic3d.h"
+#include "cuda/equations/elastic2d/elastic2d.h"
 
 namespace {
 
@@ -141,9 +142,105 @@ int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *
     return 1;
 }
 
+// Elastic 2-D (cpmls, velocity-stress): vp, vs, rho (B, nz, nx); one vz
+// source per shot (field 1), receivers recording vx and vz (fields 0, 1);
+// pml = [az, bz, azh, bzh, ax, bx, axh, bxh]; res (2, B, nrec, nt).
+int run_elastic(const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zx,
+                const float *const *pml, const float *vp, const float *vs, const float *rho, const float *grad,
+                const float *src, const int *sloc, int nsrc, const int *rloc, int nrec, const float *res,
+                float *rec_out, float *g_vp, float *g_vs, float *g_rho, char *err, int cap)
+{
+    try {
+        const int nz = grid[0], nx = grid[1];
+        const int64_t n = int64_t(nz) * nx;
+        const std::vector<int64_t> gshape{B, 1, nz, nx};
+        const int nstate = 15;
+        std::vector<std::vector<float>> store(2 * nstate + 9 + 2, std::vector<float>(size_t(B * n), 0.f));
+        std::vector<Buf> fw, aw, ws, dm;
+        for (int k = 0; k < nstate; ++k) fw.push_back(mk(store[k].data(), gshape));
+        for (int k = 0; k < nstate; ++k) aw.push_back(mk(store[nstate + k].data(), gshape));
+        for (int k = 0; k < 9; ++k) ws.push_back(mk(store[2 * nstate + k].data(), gshape));
+        for (int k = 0; k < 2; ++k) dm.push_back(mk(store[2 * nstate + 9 + k].data(), gshape));
+        std::vector<Buf> models{mk(vp, gshape), mk(vs, gshape), mk(rho, gshape)};
+        std::vector<Buf> pml_b;
+        for (int k = 0; k < 4; ++k) pml_b.push_back(mk(pml[k], {nz}));
+        for (int k = 4; k < 8; ++k) pml_b.push_back(mk(pml[k], {nx}));
+        std::vector<float> spacing{spacing_zx[1], spacing_zx[0]};
+        const int sfield = 1;
+        const int rfields[2] = {0, 1};
+        std::vector<float> u_allt(size_t(nt) * 2 * B * n, 0.f);
+        const std::vector<int64_t> ashape{nt, 2, B, nz, nx};
+        std::vector<float> lap_dummy(size_t(M + 1), 0.f);
+
+        ForwardInputCore f{};
+        f.models = list(models);
+        f.derived_models = list(dm);
+        f.source = mk(src, {B, nsrc, nt});
+        f.lap_coes = mk(lap_dummy.data(), {M + 1});
+        f.grad_coes = mk(grad, {M + 1});
+        f.M = M;
+        f.abcn = abcn;
+        f.sources_loc = mk(sloc, {B, nsrc, 2});
+        f.receivers_loc = mk(rloc, {B, nrec, 2});
+        f.source_field_indices = IntSpan{&sfield, 1};
+        f.receiver_field_indices = IntSpan{rfields, 2};
+        f.pml_vals = list(pml_b);
+        f.wavefields = list(fw);
+        f.save_all_wavefields = res != nullptr;
+        f.nt = unsigned(nt);
+        f.dt = dt;
+        f.spacing = FloatSpan{spacing.data(), 2};
+        f.record_out = mk(rec_out, {2, B, nrec, nt});
+        if (res) f.u_allt_out = mk(u_allt.data(), ashape);
+        elastic2d::forward_core(f);
+        if (!res) return 0;
+
+        std::vector<Buf> grads{mk(g_vp, gshape), mk(g_vs, gshape), mk(g_rho, gshape)};
+        BackwardInputCore bi{};
+        bi.u_forward = mk(u_allt.data(), ashape);
+        bi.adjoint_wavefields = list(aw);
+        bi.adjoint_workspace = list(ws);
+        bi.derived_models = list(dm);
+        bi.models = list(models);
+        bi.adjoint_source = mk(res, {2, B, nrec, nt});
+        bi.forward_source = mk(src, {B, nsrc, nt});
+        bi.lap_coes = f.lap_coes;
+        bi.grad_coes = f.grad_coes;
+        bi.M = M;
+        bi.abcn = abcn;
+        bi.adjoint_sources_loc = f.receivers_loc;
+        bi.forward_sources_loc = f.sources_loc;
+        bi.source_field_indices = f.source_field_indices;
+        bi.receiver_field_indices = f.receiver_field_indices;
+        bi.pml_vals = f.pml_vals;
+        bi.nt = unsigned(nt);
+        bi.dt = dt;
+        bi.spacing = f.spacing;
+        bi.compute_illumination = false;
+        bi.grads_out = list(grads);
+        elastic2d::backward_core(bi);
+        return 0;
+    } catch (const std::exception &e) {
+        put_err(e.what(), err, cap);
+    } catch (...) {
+        put_err("unknown error", err, cap);
+    }
+    return 1;
+}
+
 }  // namespace
 
 extern "C" {
+
+__attribute__((visibility("default"))) int ref_elastic_run(
+    const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zx, const float *const *pml,
+    const float *vp, const float *vs, const float *rho, const float *grad, const float *src, const int *sloc,
+    int nsrc, const int *rloc, int nrec, const float *res, float *rec_out, float *g_vp, float *g_vs, float *g_rho,
+    char *err, int cap)
+{
+    return run_elastic(grid, B, M, nt, abcn, dt, spacing_zx, pml, vp, vs, rho, grad, src, sloc, nsrc, rloc, nrec,
+                       res, rec_out, g_vp, g_vs, g_rho, err, cap);
+}
 
 __attribute__((visibility("default"))) int ref_acoustic_run(
     int ndim, const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zyx,

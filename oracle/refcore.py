@@ -31,6 +31,7 @@ def lib():
     if _lib is None:
         L = ctypes.CDLL(str(LIB))
         L.ref_acoustic_run.restype = ctypes.c_int
+        L.ref_elastic_run.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -68,3 +69,30 @@ def run(P, vp_pad, wav, sloc, rloc, res=None):
     if rc != 0:
         raise RuntimeError("reference core: " + err.value.decode(errors="replace"))
     return dict(rec=rec, grad_vp=gv, grad_wavelet=gw)
+
+
+def run_elastic(shape_rt, M, nt, abcn, dt, dh_zx, prof, vp, vs, rho, grad, src, sloc, rloc, res=None):
+    """Forward (records vx, vz) and, with ``res`` (2, B, nrec, nt), the
+    full-storage backward of the reference's elastic 2-D core.  Arrays on
+    the runtime grid (B, nz, nx); ``prof`` = [az, bz, azh, bzh, ax, bx, axh,
+    bxh]; ``grad`` the (M+1) staggered table; one vz source per shot.
+    Returns ``dict(rec (2, B, nrec, nt), grad_vp, grad_vs, grad_rho)``."""
+    B = vp.shape[0]
+    grid = np.array(shape_rt, dtype=np.int32)
+    nsrc, nrec = sloc.shape[1], rloc.shape[1]
+    pr = [_f32(p) for p in prof]
+    parr = (_F * 8)(*[_fp(p) for p in pr])
+    vp, vs, rho, grad, src = (_f32(a) for a in (vp, vs, rho, grad, src))
+    res = _f32(res) if res is not None else None
+    sloc, rloc = np.ascontiguousarray(sloc, np.int32), np.ascontiguousarray(rloc, np.int32)
+    rec = np.zeros((2, B, nrec, nt), np.float32)
+    g = [np.zeros(vp.shape, np.float32) for _ in range(3)]
+    sp = np.array(dh_zx, dtype=np.float32)
+    err = ctypes.create_string_buffer(1024)
+    rc = lib().ref_elastic_run(grid.ctypes.data_as(_I), B, M, nt, abcn, ctypes.c_float(dt), _fp(sp), parr,
+                               _fp(vp), _fp(vs), _fp(rho), _fp(grad), _fp(src), sloc.ctypes.data_as(_I), nsrc,
+                               rloc.ctypes.data_as(_I), nrec, _fp(res), _fp(rec), _fp(g[0]), _fp(g[1]), _fp(g[2]),
+                               err, 1024)
+    if rc != 0:
+        raise RuntimeError("reference core: " + err.value.decode(errors="replace"))
+    return dict(rec=rec, grad_vp=g[0], grad_vs=g[1], grad_rho=g[2])

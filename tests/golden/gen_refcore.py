@@ -46,6 +46,12 @@ CASES_2D = [
     ("o6", 6, (32, 44), 2, 2, 300),
     ("o8", 8, (32, 44), 1, 3, 300),
 ]
+CASES_E2D = [
+    # name, order, shape, B, nt
+    ("o2", 2, (36, 48), 2, 300),
+    ("o4", 4, (36, 48), 2, 300),
+    ("o8", 8, (36, 48), 2, 300),
+]
 CASES_3D = [
     ("o2", 2, (14, 12, 16), 1, 2, 150),
     ("o4", 4, (14, 12, 16), 1, 2, 150),
@@ -93,6 +99,31 @@ def main():
                                 vp=vp, wavelet=wav, sources=src, receivers=rec, residual=res,
                                 rec=out["rec"], grad_vp=out["grad_vp"], grad_wavelet=out["grad_wavelet"])
             print(path.name, os.path.getsize(path))
+    # elastic 2-D: vz sources, vx + vz receivers, layered vp with vs = vp / 1.8, Gardner rho
+    dh, dt, abcn, f0 = 5.0, 5e-4, 10, 15.0
+    for name, order, shape, B, nt in CASES_E2D:
+        rng = np.random.default_rng(3000 + order)
+        P = oracle.E2DProblem(order, shape, dh, dt, nt, abcn=abcn, B=B)
+        vp = smooth_model(rng, shape, 2500.0, 3800.0)
+        vs = (vp / 1.8 * (1.0 + 0.05 * rng.uniform(-1, 1, shape))).astype(np.float32)
+        rho = (310.0 * vp ** 0.25).astype(np.float32)
+        wav = ricker(np.arange(nt) * dt - 1.2 / f0, f0)
+        nx = shape[-1]
+        src = np.stack([rng.integers(3, nx - 3, B), rng.integers(1, 4, B)], -1)[:, None, :]
+        nrec = 9
+        rec = np.zeros((B, nrec, 2), np.int64)
+        rec[..., 0] = np.linspace(0, nx - 1, nrec).astype(np.int64)
+        rec[..., 1] = 2
+        wavB = np.broadcast_to(wav, (B, 1, nt)).copy()
+        res = rng.standard_normal((2, B, nrec, nt)).astype(np.float32)
+        out = refcore.run_elastic((P.nz, P.nx), P.M, nt, abcn, dt, (dh, dh), P.prof, P.pad_model(vp),
+                                  P.pad_model(vs), P.pad_model(rho), P.grad, wavB, P.shift(src), P.shift(rec), res)
+        path = OUT / f"refcore_e2d_{name}.npz"
+        np.savez_compressed(path, order=order, shape=np.array(shape), dh=dh, dt=dt, nt=nt, abcn=abcn, B=B,
+                            vp=vp, vs=vs, rho=rho, wavelet=wav, sources=src, receivers=rec, residual=res,
+                            rec=out["rec"], grad_vp=P.crop(out["grad_vp"]), grad_vs=P.crop(out["grad_vs"]),
+                            grad_rho=P.crop(out["grad_rho"]))
+        print(path.name, os.path.getsize(path))
 
 
 if __name__ == "__main__":

@@ -1,7 +1,8 @@
-"""The C oracle against the reference's compiled acoustic path.
+"""The C oracle against the reference's compiled acoustic and elastic paths.
 
-Fixtures ``golden/refcore{2,3}d_*.npz`` hold the reference CPU host core's
-full-storage outputs (written by ``golden/gen_refcore.py``); where
+Fixtures ``golden/refcore{2,3}d_*.npz`` and ``golden/refcore_e2d_*.npz`` hold
+the reference CPU host core's full-storage outputs (written by
+``golden/gen_refcore.py``); where
 ``oracle/_ref`` is built the same comparison also runs live on fresh inputs.
 
 Bars (relative L2, full storage): 3-D 1e-8 -- records and model gradients
@@ -33,7 +34,7 @@ from conftest import GOLDEN, rel_l2
 oracle = pytest.importorskip("oracle.oracle")
 from oracle import refcore  # noqa: E402
 
-CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(str(GOLDEN / "refcore*d_*.npz")))
+CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(str(GOLDEN / "refcore[23]d_*.npz")))
 
 
 def problem(z):
@@ -111,3 +112,65 @@ def test_oracle_matches_reference_core_live(ndim, order):
     assert rel_l2(o["rec"], ref["rec"]) <= bar
     assert rel_l2(o["grad_vp"], ref["grad_vp"]) <= bar
     assert rel_l2(o["grad_wavelet"], ref["grad_wavelet"]) <= bar
+
+
+# ---------------------------------------------------------------- elastic 2-D
+E2D_CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(str(GOLDEN / "refcore_e2d_*.npz")))
+
+
+def e2d_problem(z):
+    B, nt = int(z["B"]), int(z["nt"])
+    P = oracle.E2DProblem(int(z["order"]), tuple(int(v) for v in z["shape"]), float(z["dh"]), float(z["dt"]), nt,
+                          abcn=int(z["abcn"]), B=B)
+    wav = np.broadcast_to(np.asarray(z["wavelet"], np.float32), (B, 1, nt)).copy()
+    models = [P.pad_model(z[k]) for k in ("vp", "vs", "rho")]
+    return P, models, wav, P.shift(z["sources"]), P.shift(z["receivers"])
+
+
+def test_e2d_fixtures_present():
+    assert len(E2D_CASES) >= 3
+
+
+@pytest.mark.parametrize("case", E2D_CASES)
+def test_e2d_oracle_matches_reference_core(case):
+    """Elastic records agree to 1e-6 and the vp / vs / rho gradients to 5e-5
+    (order 2 bit-exact; the higher fixed orders differ from the reference
+    core in the rounding of a few constant products)."""
+    z = np.load(GOLDEN / f"{case}.npz")
+    P, (vp, vs, rho), wav, s, r = e2d_problem(z)
+    o = P.gradient(vp, vs, rho, wav, s, r, z["residual"])
+    assert rel_l2(o["rec"], z["rec"]) <= 1e-6
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        assert rel_l2(P.crop(o[k]), z[k]) <= 5e-5, k
+
+
+@pytest.mark.parametrize("model", [0, 1, 2], ids=["vp", "vs", "rho"])
+def test_e2d_oracle_gradient_directional_fd(model):
+    """<grad, dm> = dJ/de (J = <record, residual>) by central differences,
+    for a smooth perturbation kept away from the sources and receivers."""
+    z = np.load(GOLDEN / "refcore_e2d_o8.npz")
+    P, models, wav, s, r = e2d_problem(z)
+    res = z["residual"]
+    o = P.gradient(*models, wav, s, r, res)
+    shape = tuple(int(v) for v in z["shape"])
+    rng = np.random.default_rng(11 + model)
+    from scipy.ndimage import gaussian_filter
+    d = gaussian_filter(rng.standard_normal(shape), 3.0)
+    d[:8] = 0.0
+    d[-4:] = 0.0
+    d[:, :4] = 0.0
+    d[:, -4:] = 0.0
+    d = (d / np.abs(d).max()).astype(np.float32)
+    base = [z["vp"], z["vs"], z["rho"]][model]
+    eps = 1e-3 * float(np.abs(base).mean())
+
+    def J(sign):
+        ms = [np.asarray(z[k], np.float32).copy() for k in ("vp", "vs", "rho")]
+        ms[model] = (ms[model] + sign * eps * d).astype(np.float32)
+        rec = P.forward(*[P.pad_model(m) for m in ms], wav, s, r)
+        return float(np.sum(rec.astype(np.float64) * res))
+
+    fd = (J(1.0) - J(-1.0)) / (2 * eps)
+    key = ["grad_vp", "grad_vs", "grad_rho"][model]
+    an = float(np.sum(P.crop(o[key]).sum(0).astype(np.float64) * d))
+    assert abs(fd - an) <= 2e-3 * abs(an), (fd, an)
