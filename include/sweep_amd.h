@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWP_ABI_VERSION 1
+#define SWP_ABI_VERSION 2
 
 /* Memory strategy of a gradient run (reference: propagator/options.py
  * Full / BoundarySaving(storage='gpu') / Ckpt(mode='chunk'|'recursive')). */
@@ -70,6 +70,13 @@ typedef struct swp_problem {
     int32_t ckpt_interval;   /* SWP_MEM_CKPT: steps between snapshots */
     int32_t flags;           /* bit 0: accumulate illumination in the backward */
     int32_t reserved;
+    /* Wavefield indices the sources inject into and the receivers record
+     * (reference: source_field_indices / receiver_field_indices,
+     * csrc/core/input_core.h:19-20).  Elastic: 0 vx, 1 vz, 2 sxx, 3 szz,
+     * 4 sxz; the record and the adjoint source are (nrec_fields, B, nrec, nt).
+     * Acoustic: one field (0). */
+    int32_t nsrc_fields, nrec_fields;
+    int32_t src_fields[5], rec_fields[5];
     void *tape;              /* forward -> backward state, swp_query's tape_bytes */
     int64_t tape_bytes;
     void *workspace;         /* per-call scratch, swp_query's workspace_bytes */
@@ -78,12 +85,12 @@ typedef struct swp_problem {
 
 /* Outputs of a forward entry (kind 0). */
 typedef struct swp_forward_io {
-    float *record;           /* (B, nrec, nt), time innermost (reference raw layout) */
+    float *record;           /* ([nrec_fields,] B, nrec, nt), time innermost (reference raw layout) */
 } swp_forward_io;
 
 /* Inputs/outputs of a backward entry (kind 1).  Gradients accumulate. */
 typedef struct swp_backward_io {
-    const float *adjoint_source;  /* (B, nrec, nt) residual, raw record layout */
+    const float *adjoint_source;  /* ([nrec_fields,] B, nrec, nt) residual, raw record layout */
     float *grad_source;           /* (B, nsrc, nt) or NULL */
     float *grad_models[3];        /* (B, grid) per model, or NULL */
     float *illum_source;          /* (B, grid) sum_t u_tt^2, or NULL */

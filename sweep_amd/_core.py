@@ -15,7 +15,7 @@ import threading
 from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libswp_core.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.POINTER(ctypes.c_int32)
@@ -33,7 +33,9 @@ class SwpProblem(ctypes.Structure):
         ("models", _F * 3), ("model_batched", ctypes.c_int32), ("nsrc", ctypes.c_int32),
         ("nrec", ctypes.c_int32), ("source", _F), ("sources_loc", _I32),
         ("receivers_loc", _I32), ("memory", ctypes.c_int32), ("ckpt_interval", ctypes.c_int32),
-        ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32), ("tape", ctypes.c_void_p),
+        ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32), ("nsrc_fields", ctypes.c_int32),
+        ("nrec_fields", ctypes.c_int32), ("src_fields", ctypes.c_int32 * 5), ("rec_fields", ctypes.c_int32 * 5),
+        ("tape", ctypes.c_void_p),
         ("tape_bytes", ctypes.c_int64), ("workspace", ctypes.c_void_p),
         ("workspace_bytes", ctypes.c_int64),
     ]

@@ -56,9 +56,45 @@ struct Ctx {
     Geo g;
     Coef c;
     Prof pr;
-    bool zm;        // z-marching kernels (fixed orders 2..8)
-    int tz;         // z-segment height of the z-marching kernels
+    bool zm;             // z-marching kernels (fixed orders 2..8)
+    bool shell_percell;  // with zm: the CPML shell on the per-cell kernels
+    int tz;              // z-segment height of the z-marching kernels
 };
+
+// A box of cells [x0, x1) x [y0, y1) x [z0, z1).
+struct Box {
+    int x0, x1, y0, y1, z0, z1;
+};
+
+// Strip layout of one shot and step (ns cells): faces z-lo, z-hi as
+// (W, nyp, nxp), y-lo, y-hi as (nzp, W, nxp), x-lo, x-hi as (nzp, nyp, W);
+// a cell on an edge or corner is stored in each face that holds it.
+__device__ __forceinline__ void save_strips(const Geo &g, float *__restrict__ s, int ix, int iy, int iz, float v)
+{
+    if (ix < g.X0 || ix >= g.X1 || iy < g.Y0 || iy >= g.Y1 || iz < g.Z0 || iz >= g.Z1) return;
+    const int64_t W = g.W, nxp = g.nxp, nyp = g.nyp, nzp = g.nzp;
+    const int64_t x = ix - g.X0, y = iy - g.Y0, z = iz - g.Z0;
+    const int64_t fz = W * nyp * nxp, fy = nzp * W * nxp, fx = nzp * nyp * W;
+    if (z < W) s[(z * nyp + y) * nxp + x] = v;
+    if (iz >= g.Z1 - g.W) s[fz + ((iz - (g.Z1 - g.W)) * nyp + y) * nxp + x] = v;
+    if (y < W) s[2 * fz + (z * W + y) * nxp + x] = v;
+    if (iy >= g.Y1 - g.W) s[2 * fz + fy + (z * W + (iy - (g.Y1 - g.W))) * nxp + x] = v;
+    if (x < W) s[2 * fz + 2 * fy + (z * nyp + y) * W + x] = v;
+    if (ix >= g.X1 - g.W) s[2 * fz + 2 * fy + fx + (z * nyp + y) * W + (ix - (g.X1 - g.W))] = v;
+}
+
+__device__ __forceinline__ float load_strip(const Geo &g, const float *__restrict__ s, int ix, int iy, int iz)
+{
+    const int64_t W = g.W, nxp = g.nxp, nyp = g.nyp, nzp = g.nzp;
+    const int64_t x = ix - g.X0, y = iy - g.Y0, z = iz - g.Z0;
+    const int64_t fz = W * nyp * nxp, fy = nzp * W * nxp, fx = nzp * nyp * W;
+    if (z < W) return s[(z * nyp + y) * nxp + x];
+    if (iz >= g.Z1 - g.W) return s[fz + ((iz - (g.Z1 - g.W)) * nyp + y) * nxp + x];
+    if (y < W) return s[2 * fz + (z * W + y) * nxp + x];
+    if (iy >= g.Y1 - g.W) return s[2 * fz + fy + (z * W + (iy - (g.Y1 - g.W))) * nxp + x];
+    if (x < W) return s[2 * fz + 2 * fy + (z * nyp + y) * W + x];
+    return s[2 * fz + 2 * fy + fx + (z * nyp + y) * W + (ix - (g.X1 - g.W))];
+}
 
 bool available();
 Geo make_geo(const swp_problem &p);

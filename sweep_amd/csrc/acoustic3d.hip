@@ -70,40 +70,9 @@ Geo make_geo(const swp_problem &p)
     return g;
 }
 
-// ---------------------------------------------------------------- strips
-// Strip layout of one shot and step (ns cells): faces z-lo, z-hi as
-// (W, nyp, nxp), y-lo, y-hi as (nzp, W, nxp), x-lo, x-hi as (nzp, nyp, W);
-// a cell on an edge or corner is stored in each face that holds it.
-__device__ __forceinline__ void save_strips(const Geo &g, float *__restrict__ s, int ix, int iy, int iz, float v)
-{
-    if (ix < g.X0 || ix >= g.X1 || iy < g.Y0 || iy >= g.Y1 || iz < g.Z0 || iz >= g.Z1) return;
-    const int64_t W = g.W, nxp = g.nxp, nyp = g.nyp, nzp = g.nzp;
-    const int64_t x = ix - g.X0, y = iy - g.Y0, z = iz - g.Z0;
-    const int64_t fz = W * nyp * nxp, fy = nzp * W * nxp, fx = nzp * nyp * W;
-    if (z < W) s[(z * nyp + y) * nxp + x] = v;
-    if (iz >= g.Z1 - g.W) s[fz + ((iz - (g.Z1 - g.W)) * nyp + y) * nxp + x] = v;
-    if (y < W) s[2 * fz + (z * W + y) * nxp + x] = v;
-    if (iy >= g.Y1 - g.W) s[2 * fz + fy + (z * W + (iy - (g.Y1 - g.W))) * nxp + x] = v;
-    if (x < W) s[2 * fz + 2 * fy + (z * nyp + y) * W + x] = v;
-    if (ix >= g.X1 - g.W) s[2 * fz + 2 * fy + fx + (z * nyp + y) * W + (ix - (g.X1 - g.W))] = v;
-}
-
-__device__ __forceinline__ float load_strip(const Geo &g, const float *__restrict__ s, int ix, int iy, int iz)
-{
-    const int64_t W = g.W, nxp = g.nxp, nyp = g.nyp, nzp = g.nzp;
-    const int64_t x = ix - g.X0, y = iy - g.Y0, z = iz - g.Z0;
-    const int64_t fz = W * nyp * nxp, fy = nzp * W * nxp, fx = nzp * nyp * W;
-    if (z < W) return s[(z * nyp + y) * nxp + x];
-    if (iz >= g.Z1 - g.W) return s[fz + ((iz - (g.Z1 - g.W)) * nyp + y) * nxp + x];
-    if (y < W) return s[2 * fz + (z * W + y) * nxp + x];
-    if (iy >= g.Y1 - g.W) return s[2 * fz + fy + (z * W + (iy - (g.Y1 - g.W))) * nxp + x];
-    if (x < W) return s[2 * fz + 2 * fy + (z * nyp + y) * W + x];
-    return s[2 * fz + 2 * fy + fx + (z * nyp + y) * W + (ix - (g.X1 - g.W))];
-}
-
 // ---------------------------------------------------------------- forward
-// One thread per computed cell [M, n-M)^3 of one shot; blocks of 64 x 4
-// cells in (x, y), one z plane.  Cells inside phys_box(1) take the bare
+// One thread per cell of a box inside the computed box [M, n-M)^3 of one
+// shot; blocks of 64 x 4 cells in (x, y), one z plane.  Cells inside phys_box(1) take the bare
 // leapfrog; the shell applies, on each axis whose CPML band holds the cell
 // (pos < lo + 1 or pos >= hi - 1), the product-rule update
 //   tmp = ((1 + b) L + db du) + a dpsi + (da) psi,  w += (1 + b) tmp + a zeta,
@@ -111,15 +80,15 @@ __device__ __forceinline__ float load_strip(const Geo &g, const float *__restric
 // (forward_split.cu:132-249), the plain second derivative on the others.
 template <int MS>
 __global__ void __launch_bounds__(256) fwd_kernel(Geo g, Coef c, Prof pr, StepFields f,
-                                                  float *__restrict__ u_this, float *__restrict__ strip)
+                                                  float *__restrict__ u_this, float *__restrict__ strip, Box bx)
 {
     const int M = MS > 0 ? MS : g.M;
-    const int nzc = g.nz - 2 * M;
-    const int ix = M + int(blockIdx.x) * 64 + int(threadIdx.x);
-    const int iy = M + int(blockIdx.y) * 4 + int(threadIdx.y);
-    const int iz = M + int(blockIdx.z) % nzc;
-    const int b = int(blockIdx.z) / nzc;
-    if (ix >= g.nx - M || iy >= g.ny - M) return;
+    const int nzb = bx.z1 - bx.z0;
+    const int ix = bx.x0 + int(blockIdx.x) * 64 + int(threadIdx.x);
+    const int iy = bx.y0 + int(blockIdx.y) * 4 + int(threadIdx.y);
+    const int iz = bx.z0 + int(blockIdx.z) % nzb;
+    const int b = int(blockIdx.z) / nzb;
+    if (ix >= bx.x1 || iy >= bx.y1) return;
     const int64_t off = int64_t(b) * g.n;
     const int64_t i = (int64_t(iz) * g.ny + iy) * g.nx + ix;
     const float *__restrict__ un = f.u_now + off;
@@ -180,15 +149,15 @@ __global__ void __launch_bounds__(256) fwd_kernel(Geo g, Coef c, Prof pr, StepFi
 // fuses the lagged full-mode imaging grad += 2 dt^2 img lambda / v.
 template <int MS>
 __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFields f,
-                                                  const float *__restrict__ img, float *__restrict__ grad)
+                                                  const float *__restrict__ img, float *__restrict__ grad, Box bx)
 {
     const int M = MS > 0 ? MS : g.M;
-    const int nzc = g.nz - 2 * M;
-    const int ix = M + int(blockIdx.x) * 64 + int(threadIdx.x);
-    const int iy = M + int(blockIdx.y) * 4 + int(threadIdx.y);
-    const int iz = M + int(blockIdx.z) % nzc;
-    const int b = int(blockIdx.z) / nzc;
-    if (ix >= g.nx - M || iy >= g.ny - M) return;
+    const int nzb = bx.z1 - bx.z0;
+    const int ix = bx.x0 + int(blockIdx.x) * 64 + int(threadIdx.x);
+    const int iy = bx.y0 + int(blockIdx.y) * 4 + int(threadIdx.y);
+    const int iz = bx.z0 + int(blockIdx.z) % nzb;
+    const int b = int(blockIdx.z) / nzb;
+    if (ix >= bx.x1 || iy >= bx.y1) return;
     const int64_t off = int64_t(b) * g.n;
     const int64_t i = (int64_t(iz) * g.ny + iy) * g.nx + ix;
     const float *__restrict__ ln = f.l_now + off;
@@ -272,10 +241,6 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
 // ---------------------------------------------------------------- BS recon
 // Reverse leapfrog without PML on [lo, hi) per axis, fused u_tt imaging on
 // the cells the strip restore will not overwrite (forward_split.cu:256-328).
-struct Box {
-    int x0, x1, y0, y1, z0, z1;
-};
-
 template <int MS>
 __global__ void __launch_bounds__(256) recon_kernel(Geo g, Coef c, const float *__restrict__ vpb,
                                                     const float *__restrict__ u_prev,
@@ -425,18 +390,32 @@ static void by_order(int M, K &&k)
     }
 }
 
+static Box computed_box(const Geo &g) { return {g.M, g.nx - g.M, g.M, g.ny - g.M, g.M, g.nz - g.M}; }
+
+template <class K>
+static void per_box(const Geo &g, bool shell_only, K &&k)
+{
+    Box boxes[6];
+    int n = 1;
+    if (shell_only) n = shell_boxes(g, boxes);
+    else boxes[0] = computed_box(g);
+    for (int j = 0; j < n; ++j) {
+        const Box &b = boxes[j];
+        k(b, grid_for(b.x1 - b.x0, b.y1 - b.y0, b.z1 - b.z0, g.B));
+    }
+}
+
 static void launch_fwd(const Ctx &x, const StepFields &f, float *u_this, float *strip)
 {
     TimedScope ts(K_FWD);
     const Geo &g = x.g;
-    if (x.zm) {
-        launch_fwd_zm(x, f, u_this, strip);
-        return;
-    }
-    const dim3 grid = grid_for(g.nx - 2 * g.M, g.ny - 2 * g.M, g.nz - 2 * g.M, g.B);
+    if (x.zm) launch_fwd_zm(x, f, u_this, strip, !x.shell_percell);
+    if (x.zm && !x.shell_percell) return;
     by_order(g.M, [&](auto ms) {
-        hipLaunchKernelGGL(fwd_kernel<decltype(ms)::value>, grid, dim3(64, 4), 0, stream(), g, x.c, x.pr, f,
-                           u_this, strip);
+        per_box(g, x.zm, [&](const Box &bx, dim3 grid) {
+            hipLaunchKernelGGL(fwd_kernel<decltype(ms)::value>, grid, dim3(64, 4), 0, stream(), g, x.c, x.pr, f,
+                               u_this, strip, bx);
+        });
     });
     SWP_LAUNCHED();
 }
@@ -445,14 +424,13 @@ static void launch_adj(const Ctx &x, const AdjFields &f, const float *img, float
 {
     TimedScope ts(K_ADJ);
     const Geo &g = x.g;
-    if (x.zm) {
-        launch_adj_zm(x, f, img, grad);
-        return;
-    }
-    const dim3 grid = grid_for(g.nx - 2 * g.M, g.ny - 2 * g.M, g.nz - 2 * g.M, g.B);
+    if (x.zm) launch_adj_zm(x, f, img, grad, !x.shell_percell);
+    if (x.zm && !x.shell_percell) return;
     by_order(g.M, [&](auto ms) {
-        hipLaunchKernelGGL(adj_kernel<decltype(ms)::value>, grid, dim3(64, 4), 0, stream(), g, x.c, x.pr, f,
-                           img, grad);
+        per_box(g, x.zm, [&](const Box &bx, dim3 grid) {
+            hipLaunchKernelGGL(adj_kernel<decltype(ms)::value>, grid, dim3(64, 4), 0, stream(), g, x.c, x.pr, f,
+                               img, grad, bx);
+        });
     });
     SWP_LAUNCHED();
 }
@@ -865,6 +843,10 @@ Ctx make_ctx(const swp_problem &p)
     }
     const char *kern = std::getenv("SWP_KERNELS");
     x.zm = zm_supported(x.g) && !(kern && std::string(kern) == "percell");
+    // SWP_SHELL3=percell runs the CPML shell with the per-cell kernels over
+    // six slabs instead of the region-specialised z-marching kernels.
+    const char *sh = std::getenv("SWP_SHELL3");
+    x.shell_percell = sh && std::string(sh) == "percell";
     const char *tz = std::getenv("SWP_TZ3");
     x.tz = tz ? std::max(1, std::atoi(tz)) : 32;
     return x;

@@ -7,8 +7,13 @@ namespace swp {
 namespace a3d {
 
 bool zm_supported(const Geo &g);
-void launch_fwd_zm(const Ctx &x, const StepFields &f, float *u_this, float *strip);
-void launch_adj_zm(const Ctx &x, const AdjFields &f, const float *img, float *grad);
+// The slabs of the computed box around the lean interior [lo + M + 1,
+// hi - M - 1) per axis (the whole box when that interior is empty); returns
+// how many of out[0..5] it filled.
+int shell_boxes(const Geo &g, Box out[6]);
+// Interior z-march launch, plus the z-march shell when `shell` is set.
+void launch_fwd_zm(const Ctx &x, const StepFields &f, float *u_this, float *strip, bool shell);
+void launch_adj_zm(const Ctx &x, const AdjFields &f, const float *img, float *grad, bool shell);
 void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const float *u_now, float *u_next,
                      const float *adj, float *grad, int x0, int x1, int y0, int y1, int z0, int z1);
 

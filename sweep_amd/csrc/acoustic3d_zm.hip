@@ -1,17 +1,1080 @@
-// z-marching kernels of the 3-D acoustic path (not enabled yet: the per-cell
-// kernels of acoustic3d.hip run every order).
+// 2.5-D z-marching kernels of the 3-D acoustic path (fixed orders 2..8).
+//
+// A workgroup of 256 lanes (4 wave64s) owns a 64 (x) by TY (y) tile of
+// columns and marches down a segment of z planes.  Each lane keeps, for each
+// of its RY = TY/4 rows, the column's 2M+1 z-taps in a register queue; the
+// current (x, y) plane of the tile plus an M-cell halo on each side sits in
+// LDS for the x- and y-taps.  The LDS plane is double-buffered, so one
+// workgroup barrier per plane orders its writes against the reads of the
+// plane before.  The loads of plane z+1 (queue head, halos, the other
+// fields) are issued before plane z is computed.  Per cell that is one HBM
+// read per field read and one write per field written; halo cells come from
+// L2.
+//
+// The computed box is cut into a lean interior -- [lo + M + 1, hi - M - 1)
+// per axis: no forward CPML term, no strip cell, no adjoint CPML term (the
+// adjoint's pure region) -- run by the *_int kernels, and six slabs around
+// it run by the *_shell kernels, which evaluate each cell's CPML bands.
+// Block indices are remapped so that each XCD (workgroups go round-robin
+// over the eight) takes a contiguous run of tiles, whose halos then meet in
+// the same L2.
+//
+// The arithmetic is that of the per-cell kernels (acoustic3d.hip), which
+// follow the reference's expression trees (csrc/cuda/equations/acoustic3d/
+// forward_split.cu:81-328, adjoint_split.cu:56-318).
 #include "acoustic3d_zm.h"
+#include "fd.h"
+
+#include <algorithm>
 
 namespace swp {
 namespace a3d {
 
-bool zm_supported(const Geo &) { return false; }
-void launch_fwd_zm(const Ctx &, const StepFields &, float *, float *) { fail("acoustic3d: z-march disabled"); }
-void launch_adj_zm(const Ctx &, const AdjFields &, const float *, float *) { fail("acoustic3d: z-march disabled"); }
-void launch_recon_zm(const Ctx &, const float *, const float *, const float *, float *, const float *, float *, int,
-                     int, int, int, int, int)
+using fd::d1_taps;
+using fd::lap_taps;
+
+namespace {
+
+constexpr int kRectMax = 8;
+
+// A box of cells [x0, x1) x [y0, y1) x [z0, z1) cut into tiles of 64 x TY
+// columns (partial tiles masked) and z segments of tz planes.
+struct ZRect {
+    int x0, x1, y0, y1, z0, z1, tz;
+    int ntx, nty, nseg, nblk;
+};
+
+struct ZSet {
+    ZRect r[kRectMax];
+    int n;
+    int per_shot;   // blocks per shot
+};
+
+__device__ __forceinline__ int xcd_remap(int f, int total)
 {
-    fail("acoustic3d: z-march disabled");
+    const int q = total / 8, r = total % 8;
+    const int xcd = f % 8, local = f / 8;
+    return xcd * q + min(xcd, r) + local;
+}
+
+// This block's shot, tile origin and z segment; false for a block with no
+// work (never happens for a well-formed set).
+template <int TY>
+__device__ __forceinline__ bool zcoords(const ZSet &s, int &b, int &k, int &xo, int &yo, int &zb, int &ze)
+{
+    const int total = int(gridDim.x);
+    const int f = xcd_remap(int(blockIdx.x), total);
+    b = f / s.per_shot;
+    int t = f % s.per_shot;
+    for (k = 0; k < s.n; ++k) {
+        const ZRect &R = s.r[k];
+        if (t < R.nblk) {
+            const int tx = t % R.ntx;
+            const int ty = (t / R.ntx) % R.nty;
+            const int sg = t / (R.ntx * R.nty);
+            xo = R.x0 + 64 * tx;
+            yo = R.y0 + TY * ty;
+            zb = R.z0 + sg * R.tz;
+            ze = min(zb + R.tz, R.z1);
+            return true;
+        }
+        t -= R.nblk;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- plane I/O
+// The halo duties of a lane: y-halo rows j = w + 4t (t < YH) of the 2M rows
+// above and below the tile, x-halo columns for lanes < 2M on each of its rows.
+template <int M, int TY>
+struct PlaneMap {
+    static constexpr int LW = 64 + 2 * M;
+    static constexpr int LH = TY + 2 * M;
+    static constexpr int YH = (2 * M + 3) / 4;
+    int yh_row[YH];      // global row of each y-halo duty (clamped), or -1
+    int yh_slot[YH];     // LDS row of each duty
+    int xh_col, xh_slot; // x-halo column (clamped) and LDS column, xh_slot < 0: none
+    __device__ PlaneMap(int lane, int w, int xo, int yo, int nx, int ny)
+    {
+#pragma unroll
+        for (int t = 0; t < YH; ++t) {
+            const int j = w + 4 * t;
+            if (j < 2 * M) {
+                const int gy = j < M ? yo - M + j : yo + TY + (j - M);
+                yh_row[t] = min(max(gy, 0), ny - 1);
+                yh_slot[t] = j < M ? j : TY + M + (j - M);
+            } else {
+                yh_row[t] = -1;
+                yh_slot[t] = 0;
+            }
+        }
+        if (lane < M) {
+            xh_col = xo - M + lane;
+            xh_slot = lane;
+        } else if (lane < 2 * M) {
+            xh_col = xo + 64 + (lane - M);
+            xh_slot = 64 + M + (lane - M);
+        } else {
+            xh_col = xo;
+            xh_slot = -1;
+        }
+        xh_col = min(max(xh_col, 0), nx - 1);
+    }
+};
+
+template <int M, int RY>
+struct Lanes {
+    int x, xc;
+    bool xa;
+    int y[RY], yc[RY];
+    bool act[RY];
+    __device__ Lanes(const ZRect &R, int lane, int w, int xo, int yo, int nx, int ny)
+    {
+        x = xo + lane;
+        xa = x < R.x1;
+        xc = min(x, nx - 1);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            y[r] = yo + w + 4 * r;
+            yc[r] = min(y[r], ny - 1);
+            act[r] = xa && y[r] < R.y1;
+        }
+    }
+};
+
+// ---------------------------------------------------------------- forward
+// One template for every region of the computed box: BX/BY/BZ switch on the
+// CPML machinery of an axis (the region lies in that axis' CPML margin);
+// the lean interior runs it with all three off.  With an axis on, its psi
+// taps come from an LDS plane (x, y; halo on that axis only) or a register
+// queue (z), so each psi is read once per step; each cell then applies the
+// product-rule update on the axes whose band holds it (pos < lo + 1 or
+// pos >= hi - 1), the plain second derivative on the others, and saves its
+// strip cells.
+template <int M, int RY, bool BX, bool BY, bool BZ>
+__global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepFields f, float *__restrict__ u_this,
+                                                   float *__restrict__ strip, ZSet zs)
+{
+    constexpr bool SHELL = BX || BY || BZ;
+    constexpr int TY = 4 * RY;
+    using PM = PlaneMap<M, TY>;
+    constexpr int LW = PM::LW, PS = PM::LH * PM::LW;
+    constexpr int NPL = 1 + (BX ? 1 : 0) + (BY ? 1 : 0);
+    constexpr int SX = 1, SY = BX ? 2 : 1;   // plane slots of psi_x, psi_y
+    __shared__ float lds[2][NPL * PS];
+    int b, k, xo, yo, zb, ze;
+    if (!zcoords<TY>(zs, b, k, xo, yo, zb, ze)) return;
+    const ZRect R = zs.r[k];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nx = g.nx, ny = g.ny;
+    const int64_t sxy = g.sxy;
+    const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
+    const PM pm(lane, w, xo, yo, nx, ny);
+    const int64_t off = int64_t(b) * g.n;
+    const float *__restrict__ un = f.u_now + off;
+    const float *__restrict__ up = f.u_prev + off;
+    float *__restrict__ unx = f.u_next + off;
+    const float *__restrict__ vp = f.vp + int64_t(b) * g.mstride;
+    const float dt = g.dt;
+
+    int col[RY], xhc[RY], yhc[PM::YH];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        col[r] = L.yc[r] * nx + L.xc;
+        xhc[r] = L.yc[r] * nx + pm.xh_col;
+    }
+#pragma unroll
+    for (int t = 0; t < PM::YH; ++t) yhc[t] = (pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc;
+
+    float ax = 0.f, bx = 0.f, dbx = 0.f, dax = 0.f;
+    bool inbx = false;
+    float ay[RY], by[RY], dby[RY], day[RY];
+    bool inby[RY];
+    if constexpr (BX) {
+        const int xs = min(max(L.x, M), nx - 1 - M);
+        inbx = L.x < g.X0 + 1 || L.x >= g.X1 - 1;
+        ax = pr.a[0][xs];
+        bx = pr.b[0][xs];
+        dbx = pr.db[0][xs];
+        dax = d1_taps<M>(c, [&](int q) { return pr.a[0][xs + q]; }, g.inv_dx);
+    }
+    if constexpr (BY) {
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int ys = min(max(L.y[r], M), ny - 1 - M);
+            inby[r] = L.y[r] < g.Y0 + 1 || L.y[r] >= g.Y1 - 1;
+            ay[r] = pr.a[1][ys];
+            by[r] = pr.b[1][ys];
+            dby[r] = pr.db[1][ys];
+            day[r] = d1_taps<M>(c, [&](int q) { return pr.a[1][ys + q]; }, g.inv_dy);
+        }
+    }
+    const float *__restrict__ psx = f.psi[0] + off;
+    const float *__restrict__ psy = f.psi[1] + off;
+    const float *__restrict__ psz = f.psi[2] + off;
+    float *__restrict__ zex = f.zeta[0] + off;
+    float *__restrict__ zey = f.zeta[1] + off;
+    float *__restrict__ zez = f.zeta[2] + off;
+
+    float q[RY][2 * M + 1], qp[RY][BZ ? 2 * M + 1 : 1];
+    // plane-(z+1) registers
+    float nq[RY], nqp[RY], nup[RY], nv[RY], nxh[RY], nyh[PM::YH];
+    float npx[RY], npxh[RY], nzx[RY], npy[RY], npyh[PM::YH], nzy[RY], nzz[RY];
+
+    auto load_plane = [&](int zn) {   // everything plane zn needs except the queue heads
+        const int64_t pz = int64_t(zn) * sxy;
+        const float *U = un + pz;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            nup[r] = up[pz + col[r]];
+            nv[r] = vp[pz + col[r]];
+            nxh[r] = U[xhc[r]];
+            if constexpr (BX) {
+                npx[r] = psx[pz + col[r]];
+                npxh[r] = psx[pz + xhc[r]];
+                nzx[r] = zex[pz + col[r]];
+            }
+            if constexpr (BY) {
+                npy[r] = psy[pz + col[r]];
+                nzy[r] = zey[pz + col[r]];
+            }
+            if constexpr (BZ) nzz[r] = zez[pz + col[r]];
+        }
+#pragma unroll
+        for (int t = 0; t < PM::YH; ++t) {
+            if (pm.yh_row[t] >= 0) {
+                nyh[t] = U[yhc[t]];
+                if constexpr (BY) npyh[t] = psy[pz + yhc[t]];
+            }
+        }
+    };
+    auto store_plane = [&](float *P) {   // stage the plane whose centre sits in q[.][M]
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int row = (w + 4 * r + M) * LW;
+            P[row + lane + M] = q[r][M];
+            if (pm.xh_slot >= 0) P[row + pm.xh_slot] = nxh[r];
+            if constexpr (BX) {
+                P[SX * PS + row + lane + M] = npx[r];
+                if (pm.xh_slot >= 0) P[SX * PS + row + pm.xh_slot] = npxh[r];
+            }
+            if constexpr (BY) P[SY * PS + row + lane + M] = npy[r];
+        }
+#pragma unroll
+        for (int t = 0; t < PM::YH; ++t) {
+            if (pm.yh_row[t] >= 0) {
+                P[pm.yh_slot[t] * LW + lane + M] = nyh[t];
+                if constexpr (BY) P[SY * PS + pm.yh_slot[t] * LW + lane + M] = npyh[t];
+            }
+        }
+    };
+
+    // prologue: queues for plane zb, plane zb staged
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+#pragma unroll
+        for (int t = 0; t <= 2 * M; ++t) {
+            q[r][t] = un[int64_t(zb - M + t) * sxy + col[r]];
+            if constexpr (BZ) qp[r][t] = psz[int64_t(zb - M + t) * sxy + col[r]];
+        }
+    }
+    load_plane(zb);
+    store_plane(lds[zb & 1]);
+    __syncthreads();
+
+    for (int z = zb; z < ze; ++z) {
+        const float *P = lds[z & 1];
+        float cup[RY], cv[RY], czx[RY], czy[RY], czz[RY], cpx[RY], cpy[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            cup[r] = nup[r];
+            cv[r] = nv[r];
+            czx[r] = nzx[r];
+            czy[r] = nzy[r];
+            czz[r] = nzz[r];
+            cpx[r] = npx[r];
+            cpy[r] = npy[r];
+        }
+        {   // issue plane z+1 (clamped on the last plane; unused there)
+            const int zn = min(z + 1, ze - 1);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                nq[r] = un[int64_t(zn + M) * sxy + col[r]];
+                if constexpr (BZ) nqp[r] = psz[int64_t(zn + M) * sxy + col[r]];
+            }
+            load_plane(zn);
+        }
+        const int64_t pz = int64_t(z) * sxy;
+        bool inbz = false;
+        float az = 0.f, bz = 0.f, dbz = 0.f, daz = 0.f;
+        if constexpr (BZ) {
+            inbz = z < g.Z0 + 1 || z >= g.Z1 - 1;
+            az = pr.a[2][z];
+            bz = pr.b[2][z];
+            dbz = pr.db[2][z];
+            daz = d1_taps<M>(c, [&](int t) { return pr.a[2][z + t]; }, g.inv_dz);
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            if (!L.act[r]) continue;
+            const int row = (w + 4 * r + M) * LW + lane + M;
+            const float *Rw = P + row;
+            const float u0 = q[r][M];
+            const float lx = lap_taps<M>(c, [&](int t) { return Rw[t]; }, g.inv_dx2);
+            const float ly = lap_taps<M>(c, [&](int t) { return Rw[t * LW]; }, g.inv_dy2);
+            const float lz = lap_taps<M>(c, [&](int t) { return q[r][M + t]; }, g.inv_dz2);
+            const int64_t i = pz + col[r];
+            float w_sum;
+            if constexpr (!SHELL) {
+                w_sum = lx + ly + lz;
+            } else {
+                if (strip) save_strips(g, strip + int64_t(b) * g.ns, L.x, L.y[r], z, u0);
+                const bool bandx = BX && inbx, bandy = BY && inby[r], bandz = BZ && inbz;
+                if (!bandx && !bandy && !bandz) {
+                    w_sum = lx + ly + lz;
+                } else {
+                    w_sum = 0.f;
+                    if (bandx) {
+                        const float *Px = P + SX * PS + row;
+                        const float du = d1_taps<M>(c, [&](int t) { return Rw[t]; }, g.inv_dx);
+                        const float dpsi = d1_taps<M>(c, [&](int t) { return Px[t]; }, g.inv_dx);
+                        const float psi0 = cpx[r];
+                        const float tmp = ((1.0f + bx) * lx + dbx * du) + ax * dpsi + dax * psi0;
+                        const float z0 = czx[r];
+                        w_sum += (1.0f + bx) * tmp + ax * z0;
+                        f.psin[0][off + i] = bx * du + ax * psi0;
+                        zex[i] = bx * tmp + ax * z0;
+                    } else {
+                        w_sum += lx;
+                    }
+                    if (bandy) {
+                        const float *Py = P + SY * PS + row;
+                        const float du = d1_taps<M>(c, [&](int t) { return Rw[t * LW]; }, g.inv_dy);
+                        const float dpsi = d1_taps<M>(c, [&](int t) { return Py[t * LW]; }, g.inv_dy);
+                        const float psi0 = cpy[r];
+                        const float tmp = ((1.0f + by[r]) * ly + dby[r] * du) + ay[r] * dpsi + day[r] * psi0;
+                        const float z0 = czy[r];
+                        w_sum += (1.0f + by[r]) * tmp + ay[r] * z0;
+                        f.psin[1][off + i] = by[r] * du + ay[r] * psi0;
+                        zey[i] = by[r] * tmp + ay[r] * z0;
+                    } else {
+                        w_sum += ly;
+                    }
+                    if constexpr (BZ) {
+                        if (bandz) {
+                            const float du = d1_taps<M>(c, [&](int t) { return q[r][M + t]; }, g.inv_dz);
+                            const float dpsi = d1_taps<M>(c, [&](int t) { return qp[r][M + t]; }, g.inv_dz);
+                            const float psi0 = qp[r][M];
+                            const float tmp = ((1.0f + bz) * lz + dbz * du) + az * dpsi + daz * psi0;
+                            const float z0 = czz[r];
+                            w_sum += (1.0f + bz) * tmp + az * z0;
+                            f.psin[2][off + i] = bz * du + az * psi0;
+                            zez[i] = bz * tmp + az * z0;
+                        } else {
+                            w_sum += lz;
+                        }
+                    } else {
+                        w_sum += lz;
+                    }
+                }
+            }
+            const float v = cv[r];
+            unx[i] = 2.0f * u0 - cup[r] + (v * v) * dt * dt * w_sum;
+            if (u_this) u_this[off + i] = (v * v) * w_sum;
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+#pragma unroll
+            for (int t = 0; t < 2 * M; ++t) {
+                q[r][t] = q[r][t + 1];
+                if constexpr (BZ) qp[r][t] = qp[r][t + 1];
+            }
+            q[r][2 * M] = nq[r];
+            if constexpr (BZ) qp[r][2 * M] = nqp[r];
+        }
+        if (z + 1 < ze) store_plane(lds[(z + 1) & 1]);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- adjoint
+// The LDS plane G and the z queue hold gw = v^2 dt^2 lambda; a FIFO keeps
+// lambda for the centre.  With an axis on (the region lies in that axis'
+// adjoint band, outside phys_box(M + 1)), the transposed CPML update of
+// that axis is taken through three per-cell fields of the neighbours,
+//   T = b zeta + (1 + b) gw,  U = (1 + b) T,  V = b psi + db T,  W = a T,
+// (profiles at the neighbour's own position), staged in LDS planes (x, y)
+// or register queues (z):
+//   L = sum_k lap_k ((U(+k) - U(0)) + (U(-k) - U(0))),
+//   Dg = sum_k grad_k (V(+k) - V(-k)),  Dq = sum_k grad_k (W(+k) - W(-k)),
+//   psi' = a psi + (da) T(0) - Dq,  zeta' = a (zeta + gw)
+// -- the expression tree of the per-cell adjoint (adjoint_split.cu:56-318)
+// with each neighbour's T formed once.  img != nullptr fuses the lagged
+// full-mode imaging grad += 2 dt^2 img lambda / v.
+template <int M, int RY, bool BX, bool BY, bool BZ>
+__global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFields f, const float *__restrict__ img,
+                                                   float *__restrict__ grad, ZSet zs)
+{
+    constexpr bool SHELL = BX || BY || BZ;
+    constexpr int TY = 4 * RY;
+    using PM = PlaneMap<M, TY>;
+    constexpr int LW = PM::LW, PS = PM::LH * PM::LW;
+    constexpr int NPL = 1 + (BX ? 3 : 0) + (BY ? 3 : 0);
+    constexpr int SX = 1, SY = BX ? 4 : 1;   // first plane slot of (U, V, W) for x, y
+    __shared__ float lds[2][NPL * PS];
+    int b, k, xo, yo, zb, ze;
+    if (!zcoords<TY>(zs, b, k, xo, yo, zb, ze)) return;
+    const ZRect R = zs.r[k];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nx = g.nx, ny = g.ny;
+    const int64_t sxy = g.sxy;
+    const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
+    const PM pm(lane, w, xo, yo, nx, ny);
+    const int64_t off = int64_t(b) * g.n;
+    const float *__restrict__ ln = f.l_now + off;
+    const float *__restrict__ lp = f.l_prev + off;
+    float *__restrict__ lnx = f.l_next + off;
+    const float *__restrict__ vp = f.vp + int64_t(b) * g.mstride;
+    const float dt2 = g.dt2;
+    const float *__restrict__ psx = f.psi[0] + off;
+    const float *__restrict__ psy = f.psi[1] + off;
+    const float *__restrict__ psz = f.psi[2] + off;
+    const float *__restrict__ zex = f.zeta[0] + off;
+    const float *__restrict__ zey = f.zeta[1] + off;
+    const float *__restrict__ zez = f.zeta[2] + off;
+
+    int col[RY], xhc[RY], yhc[PM::YH];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        col[r] = L.yc[r] * nx + L.xc;
+        xhc[r] = L.yc[r] * nx + pm.xh_col;
+    }
+#pragma unroll
+    for (int t = 0; t < PM::YH; ++t) yhc[t] = (pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc;
+
+    // profiles: the lane's column / halo column (x), the row / halo rows (y)
+    float ax = 0.f, bx = 0.f, dbx = 0.f, dax = 0.f, hax = 0.f, hbx = 0.f, hdbx = 0.f;
+    float ay[RY], by[RY], dby[RY], day[RY], hay[PM::YH], hby[PM::YH], hdby[PM::YH];
+    if constexpr (BX) {
+        const int xs = min(max(L.x, M), nx - 1 - M);
+        ax = pr.a[0][L.xc];
+        bx = pr.b[0][L.xc];
+        dbx = pr.db[0][L.xc];
+        dax = d1_taps<M>(c, [&](int q) { return pr.a[0][xs + q]; }, g.inv_dx);
+        hax = pr.a[0][pm.xh_col];
+        hbx = pr.b[0][pm.xh_col];
+        hdbx = pr.db[0][pm.xh_col];
+    }
+    if constexpr (BY) {
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int ys = min(max(L.y[r], M), ny - 1 - M);
+            ay[r] = pr.a[1][L.yc[r]];
+            by[r] = pr.b[1][L.yc[r]];
+            dby[r] = pr.db[1][L.yc[r]];
+            day[r] = d1_taps<M>(c, [&](int q) { return pr.a[1][ys + q]; }, g.inv_dy);
+        }
+#pragma unroll
+        for (int t = 0; t < PM::YH; ++t) {
+            const int yr = pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0;
+            hay[t] = pr.a[1][yr];
+            hby[t] = pr.b[1][yr];
+            hdby[t] = pr.db[1][yr];
+        }
+    }
+    auto gwof = [&](float lv, float v) { return v * v * dt2 * lv; };
+
+    // queues: gw (z taps), lambda (centre FIFO), U/V/W of the z axis
+    float qg[RY][2 * M + 1], ql[RY][M + 1];
+    float qu[RY][BZ ? 2 * M + 1 : 1], qv[RY][BZ ? 2 * M + 1 : 1], qw[RY][BZ ? 2 * M + 1 : 1];
+    // plane-(z+1) registers
+    float nl[RY], nvh[RY], nlp[RY];
+    float nzh[RY], nph[RY];                                 // z-queue head raw zeta_z, psi_z
+    float nxl[RY], nxv[RY];                                 // x-halo lambda, vp
+    float nyl[PM::YH], nyv[PM::YH];                         // y-halo lambda, vp
+    float nzx[RY], npx[RY], nzxh[RY], npxh[RY];              // x: centre, x-halo zeta/psi
+    float nzy[RY], npy[RY], nzyh[PM::YH], npyh[PM::YH];      // y: centre, y-halo zeta/psi
+    float nzz[RY], npz[RY];                                 // z: centre zeta/psi
+
+    auto load_plane = [&](int zn) {
+        const int64_t pz = int64_t(zn) * sxy;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            nlp[r] = lp[pz + col[r]];
+            nxl[r] = ln[pz + xhc[r]];
+            nxv[r] = vp[pz + xhc[r]];
+            if constexpr (BX) {
+                nzx[r] = zex[pz + col[r]];
+                npx[r] = psx[pz + col[r]];
+                nzxh[r] = zex[pz + xhc[r]];
+                npxh[r] = psx[pz + xhc[r]];
+            }
+            if constexpr (BY) {
+                nzy[r] = zey[pz + col[r]];
+                npy[r] = psy[pz + col[r]];
+            }
+            if constexpr (BZ) {
+                nzz[r] = zez[pz + col[r]];
+                npz[r] = psz[pz + col[r]];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < PM::YH; ++t) {
+            if (pm.yh_row[t] >= 0) {
+                nyl[t] = ln[pz + yhc[t]];
+                nyv[t] = vp[pz + yhc[t]];
+                if constexpr (BY) {
+                    nzyh[t] = zey[pz + yhc[t]];
+                    npyh[t] = psy[pz + yhc[t]];
+                }
+            }
+        }
+    };
+    auto load_head = [&](int zh) {   // queue heads at plane zh
+        const int64_t pz = int64_t(zh) * sxy;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            nl[r] = ln[pz + col[r]];
+            nvh[r] = vp[pz + col[r]];
+            if constexpr (BZ) {
+                nzh[r] = zez[pz + col[r]];
+                nph[r] = psz[pz + col[r]];
+            }
+        }
+    };
+    // U, V, W of one cell from its raw values and its profile (b, db, a)
+    auto uvw = [&](float zeta, float psi, float gwv, float bb, float dd, float aa, float &U, float &V, float &W) {
+        const float T = bb * zeta + (1.0f + bb) * gwv;
+        U = (1.0f + bb) * T;
+        V = bb * psi + dd * T;
+        W = aa * T;
+    };
+    auto push_head = [&](int zh) {   // shift the queues, append the heads loaded for plane zh
+        float bz_ = 0.f, dbz_ = 0.f, az_ = 0.f;
+        if constexpr (BZ) {
+            bz_ = pr.b[2][zh];
+            dbz_ = pr.db[2][zh];
+            az_ = pr.a[2][zh];
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+#pragma unroll
+            for (int t = 0; t < 2 * M; ++t) {
+                qg[r][t] = qg[r][t + 1];
+                if constexpr (BZ) {
+                    qu[r][t] = qu[r][t + 1];
+                    qv[r][t] = qv[r][t + 1];
+                    qw[r][t] = qw[r][t + 1];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < M; ++t) ql[r][t] = ql[r][t + 1];
+            const float gh = gwof(nl[r], nvh[r]);
+            qg[r][2 * M] = gh;
+            ql[r][M] = nl[r];
+            if constexpr (BZ) uvw(nzh[r], nph[r], gh, bz_, dbz_, az_, qu[r][2 * M], qv[r][2 * M], qw[r][2 * M]);
+        }
+    };
+    auto store_plane = [&](float *P) {   // stage the plane whose centre gw sits in qg[.][M]
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int row = (w + 4 * r + M) * LW;
+            const float gc = qg[r][M];
+            P[row + lane + M] = gc;
+            const float gh = gwof(nxl[r], nxv[r]);
+            if (pm.xh_slot >= 0) P[row + pm.xh_slot] = gh;
+            if constexpr (BX) {
+                float U, V, W;
+                uvw(nzx[r], npx[r], gc, bx, dbx, ax, U, V, W);
+                P[SX * PS + row + lane + M] = U;
+                P[(SX + 1) * PS + row + lane + M] = V;
+                P[(SX + 2) * PS + row + lane + M] = W;
+                if (pm.xh_slot >= 0) {
+                    uvw(nzxh[r], npxh[r], gh, hbx, hdbx, hax, U, V, W);
+                    P[SX * PS + row + pm.xh_slot] = U;
+                    P[(SX + 1) * PS + row + pm.xh_slot] = V;
+                    P[(SX + 2) * PS + row + pm.xh_slot] = W;
+                }
+            }
+            if constexpr (BY) {
+                float U, V, W;
+                uvw(nzy[r], npy[r], gc, by[r], dby[r], ay[r], U, V, W);
+                P[SY * PS + row + lane + M] = U;
+                P[(SY + 1) * PS + row + lane + M] = V;
+                P[(SY + 2) * PS + row + lane + M] = W;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < PM::YH; ++t) {
+            if (pm.yh_row[t] >= 0) {
+                const int row = pm.yh_slot[t] * LW + lane + M;
+                const float gh = gwof(nyl[t], nyv[t]);
+                P[row] = gh;
+                if constexpr (BY) {
+                    float U, V, W;
+                    uvw(nzyh[t], npyh[t], gh, hby[t], hdby[t], hay[t], U, V, W);
+                    P[SY * PS + row] = U;
+                    P[(SY + 1) * PS + row] = V;
+                    P[(SY + 2) * PS + row] = W;
+                }
+            }
+        }
+    };
+
+    // prologue: queues for plane zb (heads zb-M .. zb+M), plane zb staged
+#pragma unroll
+    for (int t = 0; t <= 2 * M; ++t) {
+        load_head(zb - M + t);
+        push_head(zb - M + t);
+    }
+    load_plane(zb);
+    store_plane(lds[zb & 1]);
+    __syncthreads();
+
+    for (int z = zb; z < ze; ++z) {
+        const float *P = lds[z & 1];
+        float clp[RY], czx[RY], cpx[RY], czy[RY], cpy[RY], czz[RY], cpz[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            clp[r] = nlp[r];
+            czx[r] = nzx[r];
+            cpx[r] = npx[r];
+            czy[r] = nzy[r];
+            cpy[r] = npy[r];
+            czz[r] = nzz[r];
+            cpz[r] = npz[r];
+        }
+        {
+            const int zn = min(z + 1, ze - 1);
+            load_head(zn + M);
+            load_plane(zn);
+        }
+        const int64_t pz = int64_t(z) * sxy;
+        float az = 0.f, bz = 0.f, daz = 0.f;
+        if constexpr (BZ) {
+            az = pr.a[2][z];
+            bz = pr.b[2][z];
+            daz = d1_taps<M>(c, [&](int t) { return pr.a[2][z + t]; }, g.inv_dz);
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            if (!L.act[r]) continue;
+            const int row = (w + 4 * r + M) * LW + lane + M;
+            const float *Gw = P + row;
+            const int64_t i = pz + col[r];
+            const float gun = ql[r][0];
+            const float gw = qg[r][M];
+            if (grad && img && L.x >= g.X0 && L.x < g.X1 && L.y[r] >= g.Y0 && L.y[r] < g.Y1 && z >= g.Z0 &&
+                z < g.Z1)
+                grad[off + i] += 2.f * g.dt * g.dt * img[off + i] * gun / vp[i];
+            float out;
+            if constexpr (!SHELL) {
+                float lx = 0.f, ly = 0.f, lz = 0.f;
+#pragma unroll
+                for (int t = 1; t <= M; ++t) {
+                    lx += c.lap[t] * ((Gw[t] - gw) + (Gw[-t] - gw));
+                    ly += c.lap[t] * ((Gw[t * LW] - gw) + (Gw[-t * LW] - gw));
+                    lz += c.lap[t] * ((qg[r][M + t] - gw) + (qg[r][M - t] - gw));
+                }
+                out = 2.0f * gun - clp[r] + lx * g.inv_dx2 + ly * g.inv_dy2 + lz * g.inv_dz2;
+            } else {
+                float Lt[3], Dg[3];
+                // x
+                if constexpr (BX) {
+                    const float *Ux = P + SX * PS + row, *Vx = P + (SX + 1) * PS + row, *Wx = P + (SX + 2) * PS + row;
+                    const float gt0 = bx * czx[r] + (1.0f + bx) * gw;
+                    const float l0 = (1.0f + bx) * gt0;
+                    float Ls = 0.f, Dgg = 0.f, Dq = 0.f;
+#pragma unroll
+                    for (int t = 1; t <= M; ++t) {
+                        Ls += c.lap[t] * ((Ux[t] - l0) + (Ux[-t] - l0));
+                        Dgg += c.grad[t] * (Vx[t] - Vx[-t]);
+                        Dq += c.grad[t] * (Wx[t] - Wx[-t]);
+                    }
+                    Lt[0] = Ls * g.inv_dx2;
+                    Dg[0] = Dgg * g.inv_dx;
+                    Dq *= g.inv_dx;
+                    f.psio[0][off + i] = ax * cpx[r] + dax * gt0 - Dq;
+                    f.zetao[0][off + i] = ax * (czx[r] + gw);
+                } else {
+                    float Ls = 0.f;
+#pragma unroll
+                    for (int t = 1; t <= M; ++t) Ls += c.lap[t] * ((Gw[t] - gw) + (Gw[-t] - gw));
+                    Lt[0] = Ls * g.inv_dx2;
+                    Dg[0] = 0.f;
+                }
+                // y
+                if constexpr (BY) {
+                    const float *Uy = P + SY * PS + row, *Vy = P + (SY + 1) * PS + row, *Wy = P + (SY + 2) * PS + row;
+                    const float gt0 = by[r] * czy[r] + (1.0f + by[r]) * gw;
+                    const float l0 = (1.0f + by[r]) * gt0;
+                    float Ls = 0.f, Dgg = 0.f, Dq = 0.f;
+#pragma unroll
+                    for (int t = 1; t <= M; ++t) {
+                        Ls += c.lap[t] * ((Uy[t * LW] - l0) + (Uy[-t * LW] - l0));
+                        Dgg += c.grad[t] * (Vy[t * LW] - Vy[-t * LW]);
+                        Dq += c.grad[t] * (Wy[t * LW] - Wy[-t * LW]);
+                    }
+                    Lt[1] = Ls * g.inv_dy2;
+                    Dg[1] = Dgg * g.inv_dy;
+                    Dq *= g.inv_dy;
+                    f.psio[1][off + i] = ay[r] * cpy[r] + day[r] * gt0 - Dq;
+                    f.zetao[1][off + i] = ay[r] * (czy[r] + gw);
+                } else {
+                    float Ls = 0.f;
+#pragma unroll
+                    for (int t = 1; t <= M; ++t) Ls += c.lap[t] * ((Gw[t * LW] - gw) + (Gw[-t * LW] - gw));
+                    Lt[1] = Ls * g.inv_dy2;
+                    Dg[1] = 0.f;
+                }
+                // z
+                if constexpr (BZ) {
+                    const float gt0 = bz * czz[r] + (1.0f + bz) * gw;
+                    const float l0 = (1.0f + bz) * gt0;
+                    float Ls = 0.f, Dgg = 0.f, Dq = 0.f;
+#pragma unroll
+                    for (int t = 1; t <= M; ++t) {
+                        Ls += c.lap[t] * ((qu[r][M + t] - l0) + (qu[r][M - t] - l0));
+                        Dgg += c.grad[t] * (qv[r][M + t] - qv[r][M - t]);
+                        Dq += c.grad[t] * (qw[r][M + t] - qw[r][M - t]);
+                    }
+                    Lt[2] = Ls * g.inv_dz2;
+                    Dg[2] = Dgg * g.inv_dz;
+                    Dq *= g.inv_dz;
+                    f.psio[2][off + i] = az * cpz[r] + daz * gt0 - Dq;
+                    f.zetao[2][off + i] = az * (czz[r] + gw);
+                } else {
+                    float Ls = 0.f;
+#pragma unroll
+                    for (int t = 1; t <= M; ++t) Ls += c.lap[t] * ((qg[r][M + t] - gw) + (qg[r][M - t] - gw));
+                    Lt[2] = Ls * g.inv_dz2;
+                    Dg[2] = 0.f;
+                }
+                out = 2.0f * gun - clp[r] + (Lt[0] + Lt[1] + Lt[2] - Dg[0] - Dg[1] - Dg[2]);
+            }
+            lnx[i] = out;
+        }
+        push_head(min(z + 1, ze - 1) + M);
+        if (z + 1 < ze) store_plane(lds[(z + 1) & 1]);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- BS recon
+// Reverse leapfrog without PML over one box, fused u_tt imaging on the cells
+// of [lo + W, hi - W) per axis (the cells the strip restore leaves alone).
+template <int M, int RY>
+__global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const float *__restrict__ vpb,
+                                                       const float *__restrict__ u_prev, const float *__restrict__ u_now,
+                                                       float *__restrict__ u_next, const float *__restrict__ adj,
+                                                       float *__restrict__ grad, ZSet zs)
+{
+    constexpr int TY = 4 * RY;
+    using PM = PlaneMap<M, TY>;
+    __shared__ float lds[2][PM::LH * PM::LW];
+    int b, k, xo, yo, zb, ze;
+    if (!zcoords<TY>(zs, b, k, xo, yo, zb, ze)) return;
+    const ZRect R = zs.r[k];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nx = g.nx, ny = g.ny;
+    const int64_t sxy = g.sxy;
+    const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
+    const PM pm(lane, w, xo, yo, nx, ny);
+    const int64_t off = int64_t(b) * g.n;
+    const float *__restrict__ un = u_now + off;
+    const float *__restrict__ up = u_prev + off;
+    float *__restrict__ unx = u_next + off;
+    const float *__restrict__ vp = vpb + int64_t(b) * g.mstride;
+    const float dt = g.dt;
+    const bool img_x = L.x >= g.X0 + g.W && L.x < g.X1 - g.W;
+    bool img_y[RY];
+    int64_t col[RY], xh_row[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        col[r] = int64_t(L.yc[r]) * nx + L.xc;
+        xh_row[r] = int64_t(L.yc[r]) * nx + pm.xh_col;
+        img_y[r] = L.y[r] >= g.Y0 + g.W && L.y[r] < g.Y1 - g.W;
+    }
+    float q[RY][2 * M + 1];
+    float nq[RY], nup[RY], nv[RY], nxh[RY], na[RY], ng[RY];
+    float nyh[PM::YH];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+#pragma unroll
+        for (int t = 0; t <= 2 * M; ++t) q[r][t] = un[int64_t(zb - M + t) * sxy + col[r]];
+        nup[r] = up[int64_t(zb) * sxy + col[r]];
+        nv[r] = vp[int64_t(zb) * sxy + col[r]];
+        na[r] = grad ? adj[off + int64_t(zb) * sxy + col[r]] : 0.f;
+        ng[r] = grad ? grad[off + int64_t(zb) * sxy + col[r]] : 0.f;
+        nxh[r] = un[int64_t(zb) * sxy + xh_row[r]];
+    }
+#pragma unroll
+    for (int t = 0; t < PM::YH; ++t)
+        nyh[t] = pm.yh_row[t] >= 0 ? un[int64_t(zb) * sxy + int64_t(pm.yh_row[t]) * nx + L.xc] : 0.f;
+    {
+        float *P = lds[zb & 1];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            P[(w + 4 * r + M) * PM::LW + lane + M] = q[r][M];
+            if (pm.xh_slot >= 0) P[(w + 4 * r + M) * PM::LW + pm.xh_slot] = nxh[r];
+        }
+#pragma unroll
+        for (int t = 0; t < PM::YH; ++t)
+            if (pm.yh_row[t] >= 0) P[pm.yh_slot[t] * PM::LW + lane + M] = nyh[t];
+    }
+    __syncthreads();
+
+    for (int z = zb; z < ze; ++z) {
+        const float *P = lds[z & 1];
+        float cup[RY], cv[RY], ca[RY], cg[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            cup[r] = nup[r];
+            cv[r] = nv[r];
+            ca[r] = na[r];
+            cg[r] = ng[r];
+        }
+        {
+            const int zn = min(z + 1, ze - 1);
+            const int64_t pz = int64_t(zn) * sxy;
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                nq[r] = un[pz + int64_t(M) * sxy + col[r]];
+                nup[r] = up[pz + col[r]];
+                nv[r] = vp[pz + col[r]];
+                if (grad) {
+                    na[r] = adj[off + pz + col[r]];
+                    ng[r] = grad[off + pz + col[r]];
+                }
+                nxh[r] = un[pz + xh_row[r]];
+            }
+#pragma unroll
+            for (int t = 0; t < PM::YH; ++t)
+                if (pm.yh_row[t] >= 0) nyh[t] = un[pz + int64_t(pm.yh_row[t]) * nx + L.xc];
+        }
+        const int64_t pz = int64_t(z) * sxy;
+        const bool img_z = z >= g.Z0 + g.W && z < g.Z1 - g.W;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            if (!L.act[r]) continue;
+            const float *Rw = P + (w + 4 * r + M) * PM::LW + lane + M;
+            const float u0 = q[r][M];
+            const float w_sum = lap_taps<M>(c, [&](int t) { return Rw[t]; }, g.inv_dx2) +
+                                lap_taps<M>(c, [&](int t) { return Rw[t * PM::LW]; }, g.inv_dy2) +
+                                lap_taps<M>(c, [&](int t) { return q[r][M + t]; }, g.inv_dz2);
+            const float v = cv[r];
+            const float unxt = 2.0f * u0 - cup[r] + (v * v) * dt * dt * w_sum;
+            const int64_t i = pz + col[r];
+            unx[i] = unxt;
+            if (grad && img_x && img_y[r] && img_z) {
+                const float u_tt = (unxt - 2 * u0 + cup[r]) * g.inv_dt2;
+                grad[off + i] = cg[r] + 2.f * dt * dt * u_tt * ca[r] / v;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+#pragma unroll
+            for (int t = 0; t < 2 * M; ++t) q[r][t] = q[r][t + 1];
+            q[r][2 * M] = nq[r];
+        }
+        if (z + 1 < ze) {
+            float *Pn = lds[(z + 1) & 1];
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                Pn[(w + 4 * r + M) * PM::LW + lane + M] = q[r][M];
+                if (pm.xh_slot >= 0) Pn[(w + 4 * r + M) * PM::LW + pm.xh_slot] = nxh[r];
+            }
+#pragma unroll
+            for (int t = 0; t < PM::YH; ++t)
+                if (pm.yh_row[t] >= 0) Pn[pm.yh_slot[t] * PM::LW + lane + M] = nyh[t];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- host plans
+constexpr int kRyInt = 4;     // rows per lane of the interior kernels (tile 64 x 16)
+
+ZRect make_rect(int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY)
+{
+    ZRect R{};
+    R.x0 = x0; R.x1 = x1; R.y0 = y0; R.y1 = y1; R.z0 = z0; R.z1 = z1; R.tz = tz;
+    R.ntx = (x1 - x0 + 63) / 64;
+    R.nty = (y1 - y0 + TY - 1) / TY;
+    R.nseg = (z1 - z0 + tz - 1) / tz;
+    R.nblk = R.ntx * R.nty * R.nseg;
+    return R;
+}
+
+void add_rect(ZSet &s, int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY)
+{
+    if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
+    SWP_CHECK(s.n < kRectMax, "acoustic3d: too many block rectangles");
+    s.r[s.n] = make_rect(x0, x1, y0, y1, z0, z1, tz, TY);
+    s.per_shot += s.r[s.n].nblk;
+    ++s.n;
+}
+
+// Per axis, the computed range [M, n-M) cut at the lean interior
+// [lo + M + 1, hi - M - 1): low margin [c0, a), middle [a, b), high margin
+// [b, c1).  The margins are exactly the cells in that axis' adjoint CPML band
+// and hold its forward band and strip cells.
+struct Cuts {
+    int c0, a, b, c1;
+};
+
+Cuts cuts(int n, int M, int lo, int hi)
+{
+    Cuts k;
+    k.c0 = M;
+    k.c1 = n - M;
+    k.a = std::min(std::max(lo + M + 1, k.c0), k.c1);
+    k.b = std::min(std::max(hi - M - 1, k.a), k.c1);
+    return k;
+}
+
+// The regions whose axes lie in a margin exactly where (ex, ey, ez) say so.
+ZSet region_set(const Geo &g, bool ex, bool ey, bool ez, int tz, int TY)
+{
+    const Cuts X = cuts(g.nx, g.M, g.X0, g.X1), Y = cuts(g.ny, g.M, g.Y0, g.Y1), Z = cuts(g.nz, g.M, g.Z0, g.Z1);
+    int xs[2][2], ys[2][2], zs[2][2];
+    int nxr = 0, nyr = 0, nzr = 0;
+    auto fill = [](const Cuts &k, bool edge, int out[2][2], int &n) {
+        if (edge) {
+            out[0][0] = k.c0; out[0][1] = k.a;
+            out[1][0] = k.b; out[1][1] = k.c1;
+            n = 2;
+        } else {
+            out[0][0] = k.a; out[0][1] = k.b;
+            n = 1;
+        }
+    };
+    fill(X, ex, xs, nxr);
+    fill(Y, ey, ys, nyr);
+    fill(Z, ez, zs, nzr);
+    ZSet s{};
+    for (int i = 0; i < nzr; ++i)
+        for (int j = 0; j < nyr; ++j)
+            for (int l = 0; l < nxr; ++l)
+                add_rect(s, xs[l][0], xs[l][1], ys[j][0], ys[j][1], zs[i][0], zs[i][1], tz, TY);
+    return s;
+}
+
+template <class K>
+void by_order(int M, K &&k)
+{
+    switch (M) {
+    case 1: k(std::integral_constant<int, 1>{}); break;
+    case 2: k(std::integral_constant<int, 2>{}); break;
+    case 3: k(std::integral_constant<int, 3>{}); break;
+    case 4: k(std::integral_constant<int, 4>{}); break;
+    default: fail("acoustic3d z-march: order ", 2 * M, " not instantiated");
+    }
+}
+
+dim3 grid_of(const ZSet &s, int B)
+{
+    const int64_t n = int64_t(s.per_shot) * B;
+    SWP_CHECK(n < (int64_t(1) << 31), "acoustic3d: launch too large");
+    return dim3(unsigned(n));
+}
+
+// Rows per lane of the margin kernels, by the number of axes they carry.
+template <bool BX, bool BY, bool BZ>
+constexpr int ry_fwd() { return (int(BX) + int(BY) + int(BZ)) <= 1 ? 2 : 1; }
+template <bool BX, bool BY, bool BZ>
+constexpr int ry_adj() { return 1; }
+
+template <int M, bool BX, bool BY, bool BZ>
+void fwd_region(const Ctx &x, const StepFields &f, float *u_this, float *strip)
+{
+    constexpr bool SHELL = BX || BY || BZ;
+    constexpr int RY = SHELL ? ry_fwd<BX, BY, BZ>() : kRyInt;
+    const ZSet s = region_set(x.g, BX, BY, BZ, x.tz, 4 * RY);
+    if (!s.per_shot) return;
+    hipLaunchKernelGGL((fwd3_kernel<M, RY, BX, BY, BZ>), grid_of(s, x.g.B), dim3(256), 0, stream(), x.g, x.c, x.pr,
+                       f, u_this, strip, s);
+}
+
+template <int M, bool BX, bool BY, bool BZ>
+void adj_region(const Ctx &x, const AdjFields &f, const float *img, float *grad)
+{
+    constexpr bool SHELL = BX || BY || BZ;
+    constexpr int RY = SHELL ? ry_adj<BX, BY, BZ>() : kRyInt;
+    const ZSet s = region_set(x.g, BX, BY, BZ, x.tz, 4 * RY);
+    if (!s.per_shot) return;
+    hipLaunchKernelGGL((adj3_kernel<M, RY, BX, BY, BZ>), grid_of(s, x.g.B), dim3(256), 0, stream(), x.g, x.c, x.pr,
+                       f, img, grad, s);
+}
+
+}  // namespace
+
+bool zm_supported(const Geo &g) { return g.M >= 1 && g.M <= 4 && g.sxy < (int64_t(1) << 31); }
+
+int shell_boxes(const Geo &g, Box out[6])
+{
+    // the six slabs around the lean interior (per-cell shell launches)
+    const Cuts X = cuts(g.nx, g.M, g.X0, g.X1), Y = cuts(g.ny, g.M, g.Y0, g.Y1), Z = cuts(g.nz, g.M, g.Z0, g.Z1);
+    const Box cand[6] = {
+        {X.c0, X.c1, Y.c0, Y.c1, Z.c0, Z.a}, {X.c0, X.c1, Y.c0, Y.c1, Z.b, Z.c1},
+        {X.c0, X.c1, Y.c0, Y.a, Z.a, Z.b},   {X.c0, X.c1, Y.b, Y.c1, Z.a, Z.b},
+        {X.c0, X.a, Y.a, Y.b, Z.a, Z.b},     {X.b, X.c1, Y.a, Y.b, Z.a, Z.b},
+    };
+    int n = 0;
+    for (const Box &b : cand)
+        if (b.x1 > b.x0 && b.y1 > b.y0 && b.z1 > b.z0) out[n++] = b;
+    return n;
+}
+
+void launch_fwd_zm(const Ctx &x, const StepFields &f, float *u_this, float *strip, bool shell)
+{
+    by_order(x.g.M, [&](auto ms) {
+        constexpr int M = decltype(ms)::value;
+        if (shell) {
+            fwd_region<M, true, true, true>(x, f, u_this, strip);
+            fwd_region<M, true, true, false>(x, f, u_this, strip);
+            fwd_region<M, true, false, true>(x, f, u_this, strip);
+            fwd_region<M, false, true, true>(x, f, u_this, strip);
+            fwd_region<M, true, false, false>(x, f, u_this, strip);
+            fwd_region<M, false, true, false>(x, f, u_this, strip);
+            fwd_region<M, false, false, true>(x, f, u_this, strip);
+        }
+        fwd_region<M, false, false, false>(x, f, u_this, strip);
+    });
+    SWP_LAUNCHED();
+}
+
+void launch_adj_zm(const Ctx &x, const AdjFields &f, const float *img, float *grad, bool shell)
+{
+    by_order(x.g.M, [&](auto ms) {
+        constexpr int M = decltype(ms)::value;
+        if (shell) {
+            adj_region<M, true, true, true>(x, f, img, grad);
+            adj_region<M, true, true, false>(x, f, img, grad);
+            adj_region<M, true, false, true>(x, f, img, grad);
+            adj_region<M, false, true, true>(x, f, img, grad);
+            adj_region<M, true, false, false>(x, f, img, grad);
+            adj_region<M, false, true, false>(x, f, img, grad);
+            adj_region<M, false, false, true>(x, f, img, grad);
+        }
+        adj_region<M, false, false, false>(x, f, img, grad);
+    });
+    SWP_LAUNCHED();
+}
+
+void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const float *u_now, float *u_next,
+                     const float *adj, float *grad, int x0, int x1, int y0, int y1, int z0, int z1)
+{
+    const Geo &g = x.g;
+    ZSet s{};
+    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyInt);
+    if (!s.per_shot) return;
+    by_order(g.M, [&](auto ms) {
+        constexpr int M = decltype(ms)::value;
+        hipLaunchKernelGGL((recon_zm_kernel<M, kRyInt>), grid_of(s, g.B), dim3(256), 0, stream(), g, x.c, vp, u_prev,
+                           u_now, u_next, adj, grad, s);
+    });
+    SWP_LAUNCHED();
 }
 
 }  // namespace a3d

@@ -107,7 +107,9 @@ static const int64_t kLayout[] = {
     SWP_FIELD(swp_problem, nsrc), SWP_FIELD(swp_problem, nrec), SWP_FIELD(swp_problem, source),
     SWP_FIELD(swp_problem, sources_loc), SWP_FIELD(swp_problem, receivers_loc),
     SWP_FIELD(swp_problem, memory), SWP_FIELD(swp_problem, ckpt_interval), SWP_FIELD(swp_problem, flags),
-    SWP_FIELD(swp_problem, reserved), SWP_FIELD(swp_problem, tape), SWP_FIELD(swp_problem, tape_bytes),
+    SWP_FIELD(swp_problem, reserved), SWP_FIELD(swp_problem, nsrc_fields), SWP_FIELD(swp_problem, nrec_fields),
+    SWP_FIELD(swp_problem, src_fields), SWP_FIELD(swp_problem, rec_fields),
+    SWP_FIELD(swp_problem, tape), SWP_FIELD(swp_problem, tape_bytes),
     SWP_FIELD(swp_problem, workspace), SWP_FIELD(swp_problem, workspace_bytes),
     int64_t(sizeof(swp_forward_io)), SWP_FIELD(swp_forward_io, record),
     int64_t(sizeof(swp_backward_io)), SWP_FIELD(swp_backward_io, adjoint_source),

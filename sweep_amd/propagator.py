@@ -259,6 +259,29 @@ class PropTorch(torch.nn.Module):
             loc[..., a] += off[a]
         return loc
 
+    _FIELD_NAMES = {"vx": 0, "velocity_x": 0, "vz": 1, "velocity_z": 1, "sxx": 2, "stress_xx": 2, "szz": 3,
+                    "stress_zz": 3, "sxz": 4, "stress_xz": 4, "shear_xz": 4}
+
+    def _field_indices(self):
+        """Source / receiver wavefield indices (elastic: vx, vz, sxx, szz, sxz;
+        acoustic: the one pressure field)."""
+        if not isinstance(self.equation, Elastic):
+            return [0], [0]
+        out = []
+        for kind, names in (("source", self.source_type), ("receiver", self.receiver_type)):
+            idx = []
+            for name in names:
+                if name not in self._FIELD_NAMES:
+                    raise ValueError(f"unknown elastic {kind} field {name!r}; expected one of vx, vz, sxx, szz, sxz")
+                i = self._FIELD_NAMES[name]
+                if kind == "receiver" and i == 4:
+                    raise ValueError("sxz does not support receivers (equations/elastic.py FIELD_SPECS)")
+                idx.append(i)
+            if not 1 <= len(idx) <= 5:
+                raise ValueError(f"need 1..5 {kind} fields, got {len(idx)}")
+            out.append(idx)
+        return out[0], out[1]
+
     def crop(self, data):
         """Runtime/padded-grid array -> physical box."""
         return geometry.crop_physical(data, self.pad, self.equation.M, self.ndim)
@@ -349,6 +372,12 @@ class PropTorch(torch.nn.Module):
             P.pml[k] = _ptr(t)
         P.model_batched = 0 if shared else 1
         P.nsrc, P.nrec = nsrc, nrec
+        sf, rf = self._field_indices()
+        P.nsrc_fields, P.nrec_fields = len(sf), len(rf)
+        for k, v in enumerate(sf):
+            P.src_fields[k] = v
+        for k, v in enumerate(rf):
+            P.rec_fields[k] = v
         P.sources_loc = _iptr(src_t)
         P.receivers_loc = _iptr(rec_t)
         mem = self.memory
@@ -360,6 +389,11 @@ class PropTorch(torch.nn.Module):
             P.memory = _core.MEM_CKPT
             P.ckpt_interval = mem.interval(nt)
         requires = torch.is_grad_enabled() and (w.requires_grad or any(p.requires_grad for p in padded))
+        if requires and w.requires_grad and isinstance(eq, Elastic):
+            # the compiled elastic backward returns the model gradients only (_c.py:2368-2374)
+            raise NotImplementedError(
+                f"impl='c' {type(eq).__name__} does not return the wavelet gradient (its compiled backward "
+                "returns the model gradients only). Detach the wavelet.")
         call = _Call(eq=eq, names=tuple(f"{eq.C_NAME}_{op}" for op in
                                         ("forward", "backward", "backward_bs", "backward_ckpt",
                                          "backward_recursive_ckpt")),
