@@ -176,8 +176,16 @@ class PropTorch(torch.nn.Module):
             raise ValueError("eager_options applies to impl='eager', which this build does not have")
         if use_ckpt is True and memory is None:
             memory = Ckpt()
-        memory = as_memory_strategy(memory) if memory is not None else BoundarySaving()
+        if memory is not None:
+            memory = as_memory_strategy(memory)
+        else:
+            # the elastic core has no boundary-saving backward yet: checkpointing
+            # gives the same gradient as full storage at bounded memory
+            memory = Ckpt() if isinstance(equation, Elastic) else BoundarySaving()
         if isinstance(memory, BoundarySaving):
+            if isinstance(equation, Elastic):
+                raise NotImplementedError("Elastic with BoundarySaving is not part of this build yet; "
+                                          "use memory=Ckpt() (default) or Full().")
             memory.check_supported()
         if isinstance(memory, Ckpt) and memory.storage != "gpu":
             raise NotImplementedError("Ckpt(storage='cpu') is not part of this build.")

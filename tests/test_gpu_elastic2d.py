@@ -1,0 +1,144 @@
+"""GPU parity of the 2-D elastic path: libswp_core through PropTorch against
+the reference core's fixtures (golden/refcore_e2d_*.npz) and the oracle.
+
+Bars: records rel-L2 <= 1e-5, vp / vs / rho gradients rel-L2 <= 1e-4.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import GOLDEN, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["refcore_e2d_o2", "refcore_e2d_o4", "refcore_e2d_o8"]
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+def _memory(name):
+    from sweep_amd.options import Ckpt, Full
+    return {"full": Full(), "ckpt": Ckpt(chunks=7)}[name]
+
+
+def _prop(order, shape, dh, dt, abcn, mode, dev, src_type=None, rec_type=None):
+    from sweep_amd.equations import Elastic
+    from sweep_amd.propagator import PropTorch
+    return PropTorch(Elastic(spatial_order=order, device=dev), shape=tuple(shape), dh=dh, dt=dt, device=dev,
+                     abcn=abcn, memory=_memory(mode), source_type=src_type, receiver_type=rec_type)
+
+
+def _gpu_run(order, shape, dh, dt, abcn, mode, models, wav, src, rec, res, dev, src_type=None, rec_type=None):
+    """Forward + backward of sum(pred * res), res raw (nfield, B, nrec, nt)."""
+    solver = _prop(order, shape, dh, dt, abcn, mode, dev, src_type, rec_type)
+    ms = [torch.tensor(m, device=dev, requires_grad=True) for m in models]
+    pred = solver(torch.tensor(wav, device=dev), src, rec, models=ms)
+    canon = np.ascontiguousarray(np.asarray(res).transpose(1, 3, 2, 0))
+    loss = (pred * torch.tensor(canon, device=dev)).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    rec_raw = pred.detach().cpu().numpy().transpose(3, 0, 2, 1)
+    return dict(rec=rec_raw, grad_vp=ms[0].grad.cpu().numpy(), grad_vs=ms[1].grad.cpu().numpy(),
+                grad_rho=ms[2].grad.cpu().numpy())
+
+
+@pytest.mark.parametrize("case", FIXTURES)
+@pytest.mark.parametrize("mode", ["full", "ckpt"])
+def test_matches_reference_core_fixture(case, mode):
+    dev = _dev()
+    print(f"[elastic2d] fixture={case} mode={mode}", flush=True)
+    z = np.load(GOLDEN / f"{case}.npz")
+    got = _gpu_run(int(z["order"]), z["shape"], float(z["dh"]), float(z["dt"]), int(z["abcn"]), mode,
+                   [z["vp"], z["vs"], z["rho"]], z["wavelet"], z["sources"][:, 0, :], z["receivers"], z["residual"],
+                   dev)
+    errs = {"rec": rel_l2(got["rec"], z["rec"])}
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        errs[k] = rel_l2(got[k], z[k].sum(0))
+    print(f"[elastic2d] {case} {mode} {errs}", flush=True)
+    assert errs["rec"] <= 1e-5
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        assert errs[k] <= 1e-4, k
+
+
+def _random_case(order, seed, shape=(30, 40), abcn=8, nt=240):
+    rng = np.random.default_rng(seed)
+    from scipy.ndimage import gaussian_filter
+    v = gaussian_filter(rng.standard_normal(shape), 2.0)
+    v = (v - v.min()) / (v.max() - v.min())
+    vp = (2600.0 + 900.0 * v).astype(np.float32)
+    vs = (vp / 1.8 * (1 + 0.05 * rng.uniform(-1, 1, shape))).astype(np.float32)
+    rho = (310.0 * vp ** 0.25).astype(np.float32)
+    dt, dh = 5e-4, 5.0
+    t = np.arange(nt) * dt - 0.07
+    a = (np.pi * 18.0 * t) ** 2
+    wav = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+    B = 2
+    src = np.stack([rng.integers(3, shape[1] - 3, B), rng.integers(1, 5, B)], -1)
+    nrec = 8
+    rec = np.stack([rng.integers(0, shape[1], (B, nrec)), np.full((B, nrec), 2)], -1)
+    return dict(order=order, shape=shape, dh=dh, dt=dt, abcn=abcn, nt=nt, B=B, models=[vp, vs, rho], wav=wav,
+                src=src, rec=rec, rng=rng)
+
+
+def _oracle_run(c, src_type, rec_type, res):
+    from oracle.oracle import E2DProblem
+    P = E2DProblem(c["order"], c["shape"], c["dh"], c["dt"], c["nt"], abcn=c["abcn"], B=c["B"],
+                   source_fields=src_type, receiver_fields=rec_type)
+    wav = np.broadcast_to(c["wav"], (c["B"], 1, c["nt"])).copy()
+    o = P.gradient(*[P.pad_model(m) for m in c["models"]], wav, P.shift(c["src"][:, None, :]), P.shift(c["rec"]),
+                   res)
+    return dict(rec=o["rec"], **{k: P.crop(o[k]).sum(0) for k in ("grad_vp", "grad_vs", "grad_rho")})
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8, 10])
+@pytest.mark.parametrize("mode", ["full", "ckpt"])
+@pytest.mark.parametrize("fields", [(("vz",), ("vx", "vz")), (("vx",), ("vz", "sxx")), (("sxx",), ("szz",))],
+                         ids=["vz-vxvz", "vx-vzsxx", "sxx-szz"])
+def test_matches_oracle(order, mode, fields):
+    dev = _dev()
+    src_type, rec_type = fields
+    print(f"[elastic2d] oracle order={order} mode={mode} fields={fields}", flush=True)
+    c = _random_case(order, seed=order)
+    res = c["rng"].standard_normal((len(rec_type), c["B"], c["rec"].shape[1], c["nt"])).astype(np.float32)
+    got = _gpu_run(order, c["shape"], c["dh"], c["dt"], c["abcn"], mode, c["models"], c["wav"], c["src"], c["rec"],
+                   res, dev, list(src_type), list(rec_type))
+    ref = _oracle_run(c, src_type, rec_type, res)
+    errs = {k: rel_l2(got[k], ref[k]) for k in ("rec", "grad_vp", "grad_vs", "grad_rho")}
+    print(f"[elastic2d] {order} {mode} {fields} {errs}", flush=True)
+    assert errs["rec"] <= 1e-5
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        assert errs[k] <= 1e-4, k
+
+
+def test_ckpt_equals_full():
+    dev = _dev()
+    c = _random_case(8, seed=21)
+    res = c["rng"].standard_normal((2, c["B"], c["rec"].shape[1], c["nt"])).astype(np.float32)
+    args = (8, c["shape"], c["dh"], c["dt"], c["abcn"])
+    io = (c["models"], c["wav"], c["src"], c["rec"], res, dev)
+    full = _gpu_run(*args, "full", *io)
+    ck = _gpu_run(*args, "ckpt", *io)
+    for k in full:
+        assert np.array_equal(full[k], ck[k]), k
+
+
+def test_refusals():
+    dev = _dev()
+    from sweep_amd.options import BoundarySaving
+    from sweep_amd.equations import Elastic
+    from sweep_amd.propagator import PropTorch
+    with pytest.raises(NotImplementedError):
+        PropTorch(Elastic(spatial_order=8, device=dev), shape=(30, 40), dh=5.0, dt=5e-4, device=dev,
+                  memory=BoundarySaving())
+    c = _random_case(4, seed=3)
+    solver = _prop(4, c["shape"], c["dh"], c["dt"], c["abcn"], "ckpt", dev)
+    w = torch.tensor(c["wav"], device=dev, requires_grad=True)
+    ms = [torch.tensor(m, device=dev, requires_grad=True) for m in c["models"]]
+    with pytest.raises(NotImplementedError):
+        solver(w, c["src"], c["rec"], models=ms)
