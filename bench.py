@@ -9,6 +9,13 @@ shot batch):
   loss 0.5 ||mask (pred - obs)||^2 against the record of the true model,
   gradient w.r.t. the smoothed starting model by boundary saving.
 
+``--workload elastic2d``:
+  grid 512 x 1024 at dh = 5 m, dt = 0.5 ms, nt = 4000, 8th order, cpmls CPML
+  50 cells, seeded layered vp (1800-4500 m/s), vs = vp / 1.8 +-10%, Gardner
+  rho = 310 vp^0.25; 16 shots of a vertical-force (vz) Ricker at 12 Hz, 400
+  receivers each recording vx and vz; gradients w.r.t. vp, vs and rho by
+  checkpointing (100 snapshots, chunk replay).
+
 ``--workload acoustic3d``:
   grid (nz, ny, nx) = 256 x 512 x 512 at dh = 10 m, dt = 1 ms, nt = 2000,
   8th order, CPML 50 cells, seeded dipping-layer vp (1500-4000 m/s), 8 shots
@@ -43,10 +50,14 @@ WORKLOADS = {
                        nt=3000, order=8, abcn=50, nshots=32, f0=8.0, seed=2, per_launch=32),
     "acoustic3d": dict(name="acoustic3d_256x512x512_nt2000_8shots_bs", shape=(256, 512, 512), dh=10.0,
                        dt=1e-3, nt=2000, order=8, abcn=50, nshots=8, f0=8.0, seed=4, per_launch=2),
+    "elastic2d": dict(name="elastic2d_512x1024_nt4000_16shots_ckpt", shape=(512, 1024), dh=5.0, dt=5e-4,
+                      nt=4000, order=8, abcn=50, nshots=16, f0=12.0, seed=6, per_launch=16, nrec=400),
 }
 
 KERNEL_NAMES = {0: "fwd (forward step + strip save)", 1: "adj (exact adjoint step)",
                 2: "recon (BS reconstruction + fused imaging)"}
+KERNEL_NAMES_E2D = {0: "fwd (velocity + stress step)", 1: "adj (stress/velocity adjoint prepare + apply)",
+                    2: "image (vp/vs/rho imaging)"}
 
 
 # ----------------------------------------------------------------- problems
@@ -58,6 +69,20 @@ def make_problem(W, dev, nt):
     from sweep_amd.propagator import PropTorch
     t = np.arange(nt) * W["dt"]
     wav = synthetic.ricker(t - 1.5 / W["f0"], W["f0"])
+    if W["name"].startswith("elastic"):
+        from sweep_amd.equations import Elastic
+        from sweep_amd.options import Ckpt
+        vp_true = synthetic.layered_2d(W["shape"], seed=W["seed"], vmin=1800.0, vmax=4500.0)
+        rng = np.random.default_rng(W["seed"] + 7)
+        vs_true = (vp_true / 1.8 * (1.0 + 0.1 * rng.uniform(-1, 1, W["shape"]))).astype(np.float32)
+        rho_true = (310.0 * vp_true.astype(np.float64) ** 0.25).astype(np.float32)
+        src, rec, mask = synthetic.surface_acquisition(W["shape"], W["nshots"], seed=W["seed"],
+                                                       nrec_range=(W["nrec"], W["nrec"]))
+        solver = PropTorch(Elastic(spatial_order=W["order"], device=dev), shape=W["shape"], dh=W["dh"], dt=W["dt"],
+                           device=dev, abcn=W["abcn"], memory=Ckpt(chunks=100))
+        true = [vp_true, vs_true, rho_true]
+        init = [synthetic.smoothed(m) for m in true]
+        return solver, true, init, src, rec, None, wav
     if len(W["shape"]) == 2:
         vp_true = synthetic.layered_2d(W["shape"], seed=W["seed"])
         src, rec, mask = synthetic.surface_acquisition(W["shape"], W["nshots"], seed=W["seed"])
@@ -70,7 +95,7 @@ def make_problem(W, dev, nt):
     vp_init = synthetic.smoothed(vp_true)
     solver = PropTorch(eq, shape=W["shape"], dh=W["dh"], dt=W["dt"], device=dev, abcn=W["abcn"],
                        memory=BoundarySaving())
-    return solver, vp_true, vp_init, src, rec, mask, wav
+    return solver, [vp_true], [vp_init], src, rec, mask, wav
 
 
 def algorithmic_bytes(B, grid, M, pad, W):
@@ -111,6 +136,25 @@ def algorithmic_bytes(B, grid, M, pad, W):
             2: B * (16 * rcells + 12 * img)}
 
 
+def elastic_bytes(B, grid, M, pad):
+    """Bytes per launch class of the elastic kernels under the same rule.
+    Forward: velocity {sxx, szz, sxz, rho} -> {vx, vz} (RMW) = 32 B, stress
+    {vx, vz, lambda, mu} -> {sxx, szz, sxz} (RMW) = 40 B per computed cell,
+    +64 B per band cell (8 memory variables read and written).  Adjoint:
+    stress prepare 36 B, stress apply 32 B, velocity prepare 28 B, velocity
+    apply 40 B per cell, +64 B per band cell.  Imaging: 72 B per physical
+    cell (v(it), v(it+1), the adjoint fields, vp / vs / rho, three gradients
+    read and written)."""
+    nz, nx = grid
+    cells = (nz - 2 * M) * (nx - 2 * M)
+    allc = nz * nx
+    inner = (nz - 2 * pad - 2 * M) * (nx - 2 * pad - 2 * M)
+    band = cells - inner
+    phys = inner
+    return {0: B * (72 * cells + 64 * band), 1: B * (36 * allc + 28 * allc + 72 * cells + 64 * (allc - inner)),
+            2: B * 72 * phys}
+
+
 # ----------------------------------------------------------------- CPU leg
 def cpu_baseline(W, sample_nt):
     """The oracle (C, OpenMP) on one shot of the same grid for sample_nt steps
@@ -119,6 +163,21 @@ def cpu_baseline(W, sample_nt):
     from sweep_amd import synthetic
     t = np.arange(sample_nt) * W["dt"]
     wav = np.ascontiguousarray(synthetic.ricker(t - 1.5 / W["f0"], W["f0"])[None, None, :])
+    if W["name"].startswith("elastic"):
+        P = oracle.E2DProblem(W["order"], W["shape"], W["dh"], W["dt"], sample_nt, abcn=W["abcn"], B=1)
+        vp = synthetic.smoothed(synthetic.layered_2d(W["shape"], seed=W["seed"], vmin=1800.0, vmax=4500.0))
+        src, rec, _ = synthetic.surface_acquisition(W["shape"], 1, seed=W["seed"], nrec_range=(W["nrec"], W["nrec"]))
+        sl, rl = P.shift(src[:, None, :]), P.shift(rec)
+        res = np.random.default_rng(0).standard_normal((2, 1, rl.shape[1], sample_nt)).astype(np.float32)
+        models = [P.pad_model(vp), P.pad_model(vp / 1.8), P.pad_model(310.0 * vp ** 0.25)]
+        t0 = time.perf_counter()
+        P.gradient(*models, wav, sl, rl, res)
+        dt = time.perf_counter() - t0
+        scale = W["nt"] / sample_nt
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        return {"value": 1.0 / (dt * scale), "unit": "shot-gradients/s", "cores": cores, "kind": "port",
+                "sample": f"1 shot, {sample_nt} of {W['nt']} steps of forward + full-storage backward on the full "
+                          f"{P.nz}x{P.nx} runtime grid ({dt:.1f} s), extrapolated x{scale:g}"}
     if len(W["shape"]) == 2:
         P = oracle.A2DProblem(W["order"], W["shape"], W["dh"], W["dt"], sample_nt, abcn=W["abcn"], B=1)
         vp = P.pad_model(synthetic.smoothed(synthetic.layered_2d(W["shape"], seed=W["seed"])))
@@ -169,23 +228,23 @@ def main():
     nshots = src.shape[0]
     batches = [(b, min(b + per_launch, nshots)) for b in range(0, nshots, per_launch)]
     print(f"[bench] {W['name']} nt={nt}: building obs", file=sys.stderr, flush=True)
-    vt = torch.tensor(vp_true, device=dev)
+    vt = [torch.tensor(m, device=dev) for m in vp_true]
     with torch.no_grad():
-        obs = [solver(wav, src[a:b], rec[a:b], models=[vt]) for a, b in batches]
+        obs = [solver(wav, src[a:b], rec[a:b], models=vt) for a, b in batches]
     del vt
     masks = [None if mask is None else torch.tensor(mask[a:b], device=dev)[:, None, :, None] for a, b in batches]
-    vp0 = torch.tensor(vp_init, device=dev)
+    m0 = [torch.tensor(m, device=dev) for m in vp_init]
 
     def step():
-        vp = vp0.clone().requires_grad_(True)
+        ms = [m.clone().requires_grad_(True) for m in m0]
         for k, (a, b) in enumerate(batches):
-            pred = solver(wav, src[a:b], rec[a:b], models=[vp])
+            pred = solver(wav, src[a:b], rec[a:b], models=ms)
             r = pred - obs[k]
             if masks[k] is not None:
                 r = r * masks[k]
             loss = 0.5 * r.pow(2).sum()
             loss.backward()
-        return vp.grad
+        return ms[0].grad
 
     for k in range(args.warmup):
         step()
@@ -205,7 +264,10 @@ def main():
 
     P = solver._last_call.problem
     grid = (P.nz, P.nx) if P.ndim == 2 else (P.nz, P.ny, P.nx)
-    abytes = algorithmic_bytes(P.B, grid, P.M, W["abcn"], P.M + 1)
+    elastic = W["name"].startswith("elastic")
+    names = KERNEL_NAMES_E2D if elastic else KERNEL_NAMES
+    abytes = elastic_bytes(P.B, grid, P.M, W["abcn"]) if elastic else algorithmic_bytes(P.B, grid, P.M, W["abcn"],
+                                                                                        P.M + 1)
     per = {}
     for c in (0, 1, 2):
         ms, n = stats[c]
@@ -222,18 +284,19 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": W["name"] if args.nt is None else f"{W['name']}_nt{nt}",
                    "grid": list(W["shape"]), "runtime_grid": list(grid), "nt": nt, "shots": nshots,
-                   "shots_per_launch": per_launch, "order": W["order"], "memory": "boundary_saving"},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
+                   "shots_per_launch": per_launch, "order": W["order"],
+                   "memory": "checkpoint" if elastic else "boundary_saving"},
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": int(abytes[dom]),
                      "avg_launch_ms": per[dom]["avg_ms"],
-                     "kernels": {KERNEL_NAMES[c]: {"avg_launch_ms": per[c]["avg_ms"], "GB/s": per[c]["gbs"],
+                     "kernels": {names[c]: {"avg_launch_ms": per[c]["avg_ms"], "GB/s": per[c]["gbs"],
                                                    "frac": per[c]["gbs"] / HBM_PEAK_GBS} for c in per}},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
-        sample = args.cpu_sample_nt or (1000 if len(W["shape"]) == 2 else 4)
+        sample = args.cpu_sample_nt or (4 if len(W["shape"]) == 3 else (400 if elastic else 1000))
         out["cpu_baseline"] = cpu_baseline(W, sample)
     print(json.dumps(out), flush=True)
 

@@ -7,9 +7,10 @@
 // global load per field read and one store per field written.  The loads of
 // row z+1 are issued before row z is computed (software pipelining), so each
 // wave keeps a row of loads in flight while it computes.  The LDS row is
-// private to the wave: LDS operations of one wave complete in issue order,
-// and __builtin_amdgcn_wave_barrier keeps the compiler from moving them
-// across each other, so no workgroup barrier is needed.
+// private to the wave; a workgroup barrier (one wave: no waiting on other
+// waves) with its LDS fence orders each row's stores against the reads of
+// the row before and after -- a wave-level scheduling barrier alone does not
+// order LDS accesses, the compiler may move them across it.
 //
 // Blocks whose cells and taps stay out of every CPML band run the *_int
 // kernels (bare leapfrog, no CPML registers, tall segments); the others run
@@ -26,7 +27,7 @@
 namespace swp {
 namespace a2d {
 
-#define WAVE_SYNC() __builtin_amdgcn_wave_barrier()
+#define WAVE_SYNC() __syncthreads()
 
 // This block's strip and row segment [zb, ze).
 __device__ __forceinline__ void block_coords(const BlockSet &bs, int &strip, int &zb, int &ze)
@@ -180,15 +181,20 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
     }
     float *s_out = STRIP ? strip_out + int64_t(b) * g.ns : nullptr;
 
-    float n_q, n_pq = 0.f, n_h, n_hp = 0.f, n_px = 0.f, n_up, n_v;
+    float n_q, n_pq = 0.f, n_h, n_hp = 0.f, n_px = 0.f, n_up, n_v, n_zx = 0.f, n_zz = 0.f, n_pz = 0.f;
     auto issue = [&](int zr) {
         const int64_t rn = int64_t(zr) * nx;
         n_q = un[rn + int64_t(M) * nx + xc];
-        if (seg_z) n_pq = pr.az[zr + M] * psiz[rn + int64_t(M) * nx + xc];
+        if (seg_z) {
+            n_pq = pr.az[zr + M] * psiz[rn + int64_t(M) * nx + xc];
+            n_zz = zetaz[rn + xc];
+            n_pz = psiz[rn + xc];
+        }
         n_h = un[rn + h.col];
         if (wave_x) {
             n_hp = hax * psix[rn + h.col];
             n_px = psix[rn + xc];
+            n_zx = zetax[rn + xc];
         }
         n_up = up[rn + xc];
         n_v = vp[rn + xc];
@@ -198,7 +204,7 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
         const int64_t rz = int64_t(z) * nx;
         q[2 * M] = n_q;
         pq[2 * M] = n_pq;
-        const float hv = n_h, hp = n_hp, pxv = n_px, upv = n_up, v = n_v;
+        const float hv = n_h, hp = n_hp, pxv = n_px, upv = n_up, v = n_v, zxc = n_zx, zzc = n_zz, pzc = n_pz;
         issue(min(z + 1, ze - 1));
         WAVE_SYNC();
         row[M + lane] = q[M];
@@ -231,7 +237,7 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
                 const float dudx = d1_taps<M>(c, [&](int k) { return row[M + lane + k]; }, g.inv_dx);
                 const float dap = d1_taps<M>(c, [&](int k) { return prow[M + lane + k]; }, g.inv_dx);
                 const float tmpx = ((1.0f + bx_) * lap_x + dbx_ * dudx) + dap;
-                const float zx = zetax[rz + xc];
+                const float zx = zxc;
                 w_sum += (1.0f + bx_) * tmpx + ax_ * zx;
                 if (active) {
                     psixn[rz + x] = bx_ * dudx + ax_ * pxv;
@@ -245,10 +251,10 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
                 const float dudz = d1_taps<M>(c, [&](int k) { return q[M + k]; }, g.inv_dz);
                 const float dap = d1_taps<M>(c, [&](int k) { return pq[M + k]; }, g.inv_dz);
                 const float tmpz = ((1.0f + bz_) * lap_z + dbz_ * dudz) + dap;
-                const float zz = zetaz[rz + xc];
+                const float zz = zzc;
                 w_sum += (1.0f + bz_) * tmpz + az_ * zz;
                 if (active) {
-                    psizn[rz + x] = bz_ * dudz + az_ * psiz[rz + xc];
+                    psizn[rz + x] = bz_ * dudz + az_ * pzc;
                     zetaz[rz + x] = bz_ * tmpz + az_ * zz;
                 }
             } else {

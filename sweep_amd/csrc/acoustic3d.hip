@@ -841,14 +841,14 @@ Ctx make_ctx(const swp_problem &p)
         x.pr.b[a] = p.pml[3 * src + 1];
         x.pr.db[a] = p.pml[3 * src + 2];
     }
+    const char *tz = std::getenv("SWP_TZ3");
+    x.tz = tz ? std::max(1, std::atoi(tz)) : 32;
     const char *kern = std::getenv("SWP_KERNELS");
-    x.zm = zm_supported(x.g) && !(kern && std::string(kern) == "percell");
+    x.zm = zm_supported(x.g, x.tz) && !(kern && std::string(kern) == "percell");
     // SWP_SHELL3=percell runs the CPML shell with the per-cell kernels over
     // six slabs instead of the region-specialised z-marching kernels.
     const char *sh = std::getenv("SWP_SHELL3");
     x.shell_percell = sh && std::string(sh) == "percell";
-    const char *tz = std::getenv("SWP_TZ3");
-    x.tz = tz ? std::max(1, std::atoi(tz)) : 32;
     return x;
 }
 

@@ -6,7 +6,9 @@
 namespace swp {
 namespace a3d {
 
-bool zm_supported(const Geo &g);
+// Orders 2..8, and a z segment of tz planes plus its 2M + 1 halo and prefetch
+// planes addressable with 32-bit byte offsets (tz is clamped to fit).
+bool zm_supported(const Geo &g, int &tz);
 // The slabs of the computed box around the lean interior [lo + M + 1,
 // hi - M - 1) per axis (the whole box when that interior is empty); returns
 // how many of out[0..5] it filled.

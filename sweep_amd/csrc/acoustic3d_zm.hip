@@ -57,6 +57,40 @@ __device__ __forceinline__ int xcd_remap(int f, int total)
     return xcd * q + min(xcd, r) + local;
 }
 
+// Calls f(integral_constant<int, I>) for I = B..N-1 in order while it returns true.
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        if (f(std::integral_constant<int, I>{})) static_for<I + 1, N>(f);
+    }
+}
+
+// One shot's field seen from plane zbase on through a raw buffer resource
+// (base and extent in SGPRs).  A load or store takes the lane's in-plane byte
+// offset (a loop-invariant VGPR) and the plane's byte offset (an SGPR), so a
+// plane step costs one scalar multiply instead of a 64-bit address per lane.
+// Accesses past the field's end read 0 and write nothing.  The caller keeps
+// (planes used) * sxy * 4 below 2^32.
+struct Fld {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Fld(const float *field, int64_t base_cell, int64_t end_cell)
+    {
+        const int64_t bytes = (end_cell - base_cell) * 4;
+        r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(field + base_cell), short(0),
+                                              int(uint32_t(bytes < 0xffffffffLL ? bytes : 0xffffffffLL)),
+                                              0x00020000);   // gfx9 dword 3: 32-bit raw data
+    }
+    __device__ __forceinline__ float ld(uint32_t voff, uint32_t soff) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    }
+    __device__ __forceinline__ void st(float v, uint32_t voff, uint32_t soff) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+    }
+};
+
 // This block's shot, tile origin and z segment; false for a block with no
 // work (never happens for a well-formed set).
 template <int TY>
@@ -171,20 +205,23 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
     const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
     const int64_t off = int64_t(b) * g.n;
-    const float *__restrict__ un = f.u_now + off;
-    const float *__restrict__ up = f.u_prev + off;
-    float *__restrict__ unx = f.u_next + off;
-    const float *__restrict__ vp = f.vp + int64_t(b) * g.mstride;
     const float dt = g.dt;
+    // fields from plane zb - M on; plane z sits at byte offset pl(z)
+    const int zbase = zb - M;
+    const int64_t fb = off + int64_t(zbase) * sxy, fe = off + g.n;
+    const int64_t mb = int64_t(b) * g.mstride + int64_t(zbase) * sxy, me = int64_t(b) * g.mstride + g.n;
+    const uint32_t sxy4 = uint32_t(sxy) * 4u;
+    auto pl = [&](int z) { return uint32_t(z - zbase) * sxy4; };
+    const Fld un(f.u_now, fb, fe), up(f.u_prev, fb, fe), unx(f.u_next, fb, fe), vp(f.vp, mb, me);
 
-    int col[RY], xhc[RY], yhc[PM::YH];
+    uint32_t col[RY], xhc[RY], yhc[PM::YH];   // in-plane byte offsets
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-        col[r] = L.yc[r] * nx + L.xc;
-        xhc[r] = L.yc[r] * nx + pm.xh_col;
+        col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
+        xhc[r] = uint32_t(L.yc[r] * nx + pm.xh_col) * 4u;
     }
 #pragma unroll
-    for (int t = 0; t < PM::YH; ++t) yhc[t] = (pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc;
+    for (int t = 0; t < PM::YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
 
     float ax = 0.f, bx = 0.f, dbx = 0.f, dax = 0.f;
     bool inbx = false;
@@ -209,12 +246,9 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
             day[r] = d1_taps<M>(c, [&](int q) { return pr.a[1][ys + q]; }, g.inv_dy);
         }
     }
-    const float *__restrict__ psx = f.psi[0] + off;
-    const float *__restrict__ psy = f.psi[1] + off;
-    const float *__restrict__ psz = f.psi[2] + off;
-    float *__restrict__ zex = f.zeta[0] + off;
-    float *__restrict__ zey = f.zeta[1] + off;
-    float *__restrict__ zez = f.zeta[2] + off;
+    const Fld psx(f.psi[0], fb, fe), psy(f.psi[1], fb, fe), psz(f.psi[2], fb, fe);
+    const Fld pnx(f.psin[0], fb, fe), pny(f.psin[1], fb, fe), pnz(f.psin[2], fb, fe);
+    const Fld zex(f.zeta[0], fb, fe), zey(f.zeta[1], fb, fe), zez(f.zeta[2], fb, fe);
 
     float q[RY][2 * M + 1], qp[RY][BZ ? 2 * M + 1 : 1];
     // plane-(z+1) registers
@@ -222,37 +256,36 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
     float npx[RY], npxh[RY], nzx[RY], npy[RY], npyh[PM::YH], nzy[RY], nzz[RY];
 
     auto load_plane = [&](int zn) {   // everything plane zn needs except the queue heads
-        const int64_t pz = int64_t(zn) * sxy;
-        const float *U = un + pz;
+        const uint32_t so = pl(zn);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            nup[r] = up[pz + col[r]];
-            nv[r] = vp[pz + col[r]];
-            nxh[r] = U[xhc[r]];
+            nup[r] = up.ld(col[r], so);
+            nv[r] = vp.ld(col[r], so);
+            nxh[r] = un.ld(xhc[r], so);
             if constexpr (BX) {
-                npx[r] = psx[pz + col[r]];
-                npxh[r] = psx[pz + xhc[r]];
-                nzx[r] = zex[pz + col[r]];
+                npx[r] = psx.ld(col[r], so);
+                npxh[r] = psx.ld(xhc[r], so);
+                nzx[r] = zex.ld(col[r], so);
             }
             if constexpr (BY) {
-                npy[r] = psy[pz + col[r]];
-                nzy[r] = zey[pz + col[r]];
+                npy[r] = psy.ld(col[r], so);
+                nzy[r] = zey.ld(col[r], so);
             }
-            if constexpr (BZ) nzz[r] = zez[pz + col[r]];
+            if constexpr (BZ) nzz[r] = zez.ld(col[r], so);
         }
 #pragma unroll
         for (int t = 0; t < PM::YH; ++t) {
             if (pm.yh_row[t] >= 0) {
-                nyh[t] = U[yhc[t]];
-                if constexpr (BY) npyh[t] = psy[pz + yhc[t]];
+                nyh[t] = un.ld(yhc[t], so);
+                if constexpr (BY) npyh[t] = psy.ld(yhc[t], so);
             }
         }
     };
-    auto store_plane = [&](float *P) {   // stage the plane whose centre sits in q[.][M]
+    auto store_plane = [&](float *P, int cs) {   // stage the plane whose centre sits in queue slot cs
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             const int row = (w + 4 * r + M) * LW;
-            P[row + lane + M] = q[r][M];
+            P[row + lane + M] = q[r][cs];
             if (pm.xh_slot >= 0) P[row + pm.xh_slot] = nxh[r];
             if constexpr (BX) {
                 P[SX * PS + row + lane + M] = npx[r];
@@ -269,20 +302,29 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
         }
     };
 
-    // prologue: queues for plane zb, plane zb staged
+    // prologue: queue slots t = 0..2M hold planes zb - M + t, plane zb staged
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
 #pragma unroll
         for (int t = 0; t <= 2 * M; ++t) {
-            q[r][t] = un[int64_t(zb - M + t) * sxy + col[r]];
-            if constexpr (BZ) qp[r][t] = psz[int64_t(zb - M + t) * sxy + col[r]];
+            q[r][t] = un.ld(col[r], pl(zb - M + t));
+            if constexpr (BZ) qp[r][t] = psz.ld(col[r], pl(zb - M + t));
         }
     }
     load_plane(zb);
-    store_plane(lds[zb & 1]);
+    store_plane(lds[zb & 1], M);
     __syncthreads();
 
-    for (int z = zb; z < ze; ++z) {
+    // The z loop runs in rounds of Q = 2M+1 planes with the queue rotation
+    // resolved at compile time: at step S of a round, plane z + d sits in slot
+    // (S + M + d) mod Q, and the slot of plane z - M takes plane z + M + 1.
+    constexpr int Q = 2 * M + 1;
+    for (int z0 = zb; z0 < ze; z0 += Q) {
+        static_for<0, Q>([&](auto Sc) -> bool {
+        constexpr int S = decltype(Sc)::value;
+        constexpr int C = (S + M) % Q;
+        const int z = z0 + S;
+        if (z >= ze) return false;
         const float *P = lds[z & 1];
         float cup[RY], cv[RY], czx[RY], czy[RY], czz[RY], cpx[RY], cpy[RY];
 #pragma unroll
@@ -297,14 +339,15 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
         }
         {   // issue plane z+1 (clamped on the last plane; unused there)
             const int zn = min(z + 1, ze - 1);
+            const uint32_t so = pl(zn + M);
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
-                nq[r] = un[int64_t(zn + M) * sxy + col[r]];
-                if constexpr (BZ) nqp[r] = psz[int64_t(zn + M) * sxy + col[r]];
+                nq[r] = un.ld(col[r], so);
+                if constexpr (BZ) nqp[r] = psz.ld(col[r], so);
             }
             load_plane(zn);
         }
-        const int64_t pz = int64_t(z) * sxy;
+        const uint32_t so = pl(z);
         bool inbz = false;
         float az = 0.f, bz = 0.f, dbz = 0.f, daz = 0.f;
         if constexpr (BZ) {
@@ -319,11 +362,10 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
             if (!L.act[r]) continue;
             const int row = (w + 4 * r + M) * LW + lane + M;
             const float *Rw = P + row;
-            const float u0 = q[r][M];
+            const float u0 = q[r][C];
             const float lx = lap_taps<M>(c, [&](int t) { return Rw[t]; }, g.inv_dx2);
             const float ly = lap_taps<M>(c, [&](int t) { return Rw[t * LW]; }, g.inv_dy2);
-            const float lz = lap_taps<M>(c, [&](int t) { return q[r][M + t]; }, g.inv_dz2);
-            const int64_t i = pz + col[r];
+            const float lz = lap_taps<M>(c, [&](int t) { return q[r][(C + t + Q) % Q]; }, g.inv_dz2);
             float w_sum;
             if constexpr (!SHELL) {
                 w_sum = lx + ly + lz;
@@ -342,8 +384,8 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
                         const float tmp = ((1.0f + bx) * lx + dbx * du) + ax * dpsi + dax * psi0;
                         const float z0 = czx[r];
                         w_sum += (1.0f + bx) * tmp + ax * z0;
-                        f.psin[0][off + i] = bx * du + ax * psi0;
-                        zex[i] = bx * tmp + ax * z0;
+                        pnx.st(bx * du + ax * psi0, col[r], so);
+                        zex.st(bx * tmp + ax * z0, col[r], so);
                     } else {
                         w_sum += lx;
                     }
@@ -355,21 +397,21 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
                         const float tmp = ((1.0f + by[r]) * ly + dby[r] * du) + ay[r] * dpsi + day[r] * psi0;
                         const float z0 = czy[r];
                         w_sum += (1.0f + by[r]) * tmp + ay[r] * z0;
-                        f.psin[1][off + i] = by[r] * du + ay[r] * psi0;
-                        zey[i] = by[r] * tmp + ay[r] * z0;
+                        pny.st(by[r] * du + ay[r] * psi0, col[r], so);
+                        zey.st(by[r] * tmp + ay[r] * z0, col[r], so);
                     } else {
                         w_sum += ly;
                     }
                     if constexpr (BZ) {
                         if (bandz) {
-                            const float du = d1_taps<M>(c, [&](int t) { return q[r][M + t]; }, g.inv_dz);
-                            const float dpsi = d1_taps<M>(c, [&](int t) { return qp[r][M + t]; }, g.inv_dz);
-                            const float psi0 = qp[r][M];
+                            const float du = d1_taps<M>(c, [&](int t) { return q[r][(C + t + Q) % Q]; }, g.inv_dz);
+                            const float dpsi = d1_taps<M>(c, [&](int t) { return qp[r][(C + t + Q) % Q]; }, g.inv_dz);
+                            const float psi0 = qp[r][C];
                             const float tmp = ((1.0f + bz) * lz + dbz * du) + az * dpsi + daz * psi0;
                             const float z0 = czz[r];
                             w_sum += (1.0f + bz) * tmp + az * z0;
-                            f.psin[2][off + i] = bz * du + az * psi0;
-                            zez[i] = bz * tmp + az * z0;
+                            pnz.st(bz * du + az * psi0, col[r], so);
+                            zez.st(bz * tmp + az * z0, col[r], so);
                         } else {
                             w_sum += lz;
                         }
@@ -379,21 +421,18 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
                 }
             }
             const float v = cv[r];
-            unx[i] = 2.0f * u0 - cup[r] + (v * v) * dt * dt * w_sum;
-            if (u_this) u_this[off + i] = (v * v) * w_sum;
+            unx.st(2.0f * u0 - cup[r] + (v * v) * dt * dt * w_sum, col[r], so);
+            if (u_this) u_this[off + int64_t(z) * sxy + col[r] / 4u] = (v * v) * w_sum;
         }
 #pragma unroll
-        for (int r = 0; r < RY; ++r) {
-#pragma unroll
-            for (int t = 0; t < 2 * M; ++t) {
-                q[r][t] = q[r][t + 1];
-                if constexpr (BZ) qp[r][t] = qp[r][t + 1];
-            }
-            q[r][2 * M] = nq[r];
-            if constexpr (BZ) qp[r][2 * M] = nqp[r];
+        for (int r = 0; r < RY; ++r) {   // plane z-M leaves slot S, plane z+M+1 enters it
+            q[r][S] = nq[r];
+            if constexpr (BZ) qp[r][S] = nqp[r];
         }
-        if (z + 1 < ze) store_plane(lds[(z + 1) & 1]);
+        if (z + 1 < ze) store_plane(lds[(z + 1) & 1], (C + 1) % Q);
         __syncthreads();
+        return true;
+            });
     }
 }
 
@@ -431,26 +470,27 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
     const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
     const int64_t off = int64_t(b) * g.n;
-    const float *__restrict__ ln = f.l_now + off;
-    const float *__restrict__ lp = f.l_prev + off;
-    float *__restrict__ lnx = f.l_next + off;
-    const float *__restrict__ vp = f.vp + int64_t(b) * g.mstride;
     const float dt2 = g.dt2;
-    const float *__restrict__ psx = f.psi[0] + off;
-    const float *__restrict__ psy = f.psi[1] + off;
-    const float *__restrict__ psz = f.psi[2] + off;
-    const float *__restrict__ zex = f.zeta[0] + off;
-    const float *__restrict__ zey = f.zeta[1] + off;
-    const float *__restrict__ zez = f.zeta[2] + off;
+    // fields from plane zb - M on; plane z sits at byte offset pl(z)
+    const int zbase = zb - M;
+    const int64_t fb = off + int64_t(zbase) * sxy, fe = off + g.n;
+    const int64_t mb = int64_t(b) * g.mstride + int64_t(zbase) * sxy, me = int64_t(b) * g.mstride + g.n;
+    const uint32_t sxy4 = uint32_t(sxy) * 4u;
+    auto pl = [&](int z) { return uint32_t(z - zbase) * sxy4; };
+    const Fld ln(f.l_now, fb, fe), lp(f.l_prev, fb, fe), lnx(f.l_next, fb, fe), vp(f.vp, mb, me);
+    const Fld psx(f.psi[0], fb, fe), psy(f.psi[1], fb, fe), psz(f.psi[2], fb, fe);
+    const Fld zex(f.zeta[0], fb, fe), zey(f.zeta[1], fb, fe), zez(f.zeta[2], fb, fe);
+    const Fld pox(f.psio[0], fb, fe), poy(f.psio[1], fb, fe), poz(f.psio[2], fb, fe);
+    const Fld zox(f.zetao[0], fb, fe), zoy(f.zetao[1], fb, fe), zoz(f.zetao[2], fb, fe);
 
-    int col[RY], xhc[RY], yhc[PM::YH];
+    uint32_t col[RY], xhc[RY], yhc[PM::YH];   // in-plane byte offsets
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-        col[r] = L.yc[r] * nx + L.xc;
-        xhc[r] = L.yc[r] * nx + pm.xh_col;
+        col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
+        xhc[r] = uint32_t(L.yc[r] * nx + pm.xh_col) * 4u;
     }
 #pragma unroll
-    for (int t = 0; t < PM::YH; ++t) yhc[t] = (pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc;
+    for (int t = 0; t < PM::YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
 
     // profiles: the lane's column / halo column (x), the row / halo rows (y)
     float ax = 0.f, bx = 0.f, dbx = 0.f, dax = 0.f, hax = 0.f, hbx = 0.f, hdbx = 0.f;
@@ -484,8 +524,9 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
     }
     auto gwof = [&](float lv, float v) { return v * v * dt2 * lv; };
 
-    // queues: gw (z taps), lambda (centre FIFO), U/V/W of the z axis
-    float qg[RY][2 * M + 1], ql[RY][M + 1];
+    // queues over planes z-M..z+M: gw (z taps), lambda (centre), U/V/W of the z axis
+    constexpr int Q = 2 * M + 1;
+    float qg[RY][Q], ql[RY][Q];
     float qu[RY][BZ ? 2 * M + 1 : 1], qv[RY][BZ ? 2 * M + 1 : 1], qw[RY][BZ ? 2 * M + 1 : 1];
     // plane-(z+1) registers
     float nl[RY], nvh[RY], nlp[RY];
@@ -497,48 +538,48 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
     float nzz[RY], npz[RY];                                 // z: centre zeta/psi
 
     auto load_plane = [&](int zn) {
-        const int64_t pz = int64_t(zn) * sxy;
+        const uint32_t so = pl(zn);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            nlp[r] = lp[pz + col[r]];
-            nxl[r] = ln[pz + xhc[r]];
-            nxv[r] = vp[pz + xhc[r]];
+            nlp[r] = lp.ld(col[r], so);
+            nxl[r] = ln.ld(xhc[r], so);
+            nxv[r] = vp.ld(xhc[r], so);
             if constexpr (BX) {
-                nzx[r] = zex[pz + col[r]];
-                npx[r] = psx[pz + col[r]];
-                nzxh[r] = zex[pz + xhc[r]];
-                npxh[r] = psx[pz + xhc[r]];
+                nzx[r] = zex.ld(col[r], so);
+                npx[r] = psx.ld(col[r], so);
+                nzxh[r] = zex.ld(xhc[r], so);
+                npxh[r] = psx.ld(xhc[r], so);
             }
             if constexpr (BY) {
-                nzy[r] = zey[pz + col[r]];
-                npy[r] = psy[pz + col[r]];
+                nzy[r] = zey.ld(col[r], so);
+                npy[r] = psy.ld(col[r], so);
             }
             if constexpr (BZ) {
-                nzz[r] = zez[pz + col[r]];
-                npz[r] = psz[pz + col[r]];
+                nzz[r] = zez.ld(col[r], so);
+                npz[r] = psz.ld(col[r], so);
             }
         }
 #pragma unroll
         for (int t = 0; t < PM::YH; ++t) {
             if (pm.yh_row[t] >= 0) {
-                nyl[t] = ln[pz + yhc[t]];
-                nyv[t] = vp[pz + yhc[t]];
+                nyl[t] = ln.ld(yhc[t], so);
+                nyv[t] = vp.ld(yhc[t], so);
                 if constexpr (BY) {
-                    nzyh[t] = zey[pz + yhc[t]];
-                    npyh[t] = psy[pz + yhc[t]];
+                    nzyh[t] = zey.ld(yhc[t], so);
+                    npyh[t] = psy.ld(yhc[t], so);
                 }
             }
         }
     };
     auto load_head = [&](int zh) {   // queue heads at plane zh
-        const int64_t pz = int64_t(zh) * sxy;
+        const uint32_t so = pl(zh);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            nl[r] = ln[pz + col[r]];
-            nvh[r] = vp[pz + col[r]];
+            nl[r] = ln.ld(col[r], so);
+            nvh[r] = vp.ld(col[r], so);
             if constexpr (BZ) {
-                nzh[r] = zez[pz + col[r]];
-                nph[r] = psz[pz + col[r]];
+                nzh[r] = zez.ld(col[r], so);
+                nph[r] = psz.ld(col[r], so);
             }
         }
     };
@@ -549,7 +590,7 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
         V = bb * psi + dd * T;
         W = aa * T;
     };
-    auto push_head = [&](int zh) {   // shift the queues, append the heads loaded for plane zh
+    auto push_head = [&](int zh, int slot) {   // the heads loaded for plane zh enter queue slot `slot`
         float bz_ = 0.f, dbz_ = 0.f, az_ = 0.f;
         if constexpr (BZ) {
             bz_ = pr.b[2][zh];
@@ -558,28 +599,17 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
         }
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-#pragma unroll
-            for (int t = 0; t < 2 * M; ++t) {
-                qg[r][t] = qg[r][t + 1];
-                if constexpr (BZ) {
-                    qu[r][t] = qu[r][t + 1];
-                    qv[r][t] = qv[r][t + 1];
-                    qw[r][t] = qw[r][t + 1];
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < M; ++t) ql[r][t] = ql[r][t + 1];
             const float gh = gwof(nl[r], nvh[r]);
-            qg[r][2 * M] = gh;
-            ql[r][M] = nl[r];
-            if constexpr (BZ) uvw(nzh[r], nph[r], gh, bz_, dbz_, az_, qu[r][2 * M], qv[r][2 * M], qw[r][2 * M]);
+            qg[r][slot] = gh;
+            ql[r][slot] = nl[r];
+            if constexpr (BZ) uvw(nzh[r], nph[r], gh, bz_, dbz_, az_, qu[r][slot], qv[r][slot], qw[r][slot]);
         }
     };
-    auto store_plane = [&](float *P) {   // stage the plane whose centre gw sits in qg[.][M]
+    auto store_plane = [&](float *P, int cs) {   // stage the plane whose centre gw sits in queue slot cs
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             const int row = (w + 4 * r + M) * LW;
-            const float gc = qg[r][M];
+            const float gc = qg[r][cs];
             P[row + lane + M] = gc;
             const float gh = gwof(nxl[r], nxv[r]);
             if (pm.xh_slot >= 0) P[row + pm.xh_slot] = gh;
@@ -621,17 +651,24 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
         }
     };
 
-    // prologue: queues for plane zb (heads zb-M .. zb+M), plane zb staged
+    // prologue: queue slot t holds plane zb - M + t, plane zb staged
 #pragma unroll
     for (int t = 0; t <= 2 * M; ++t) {
         load_head(zb - M + t);
-        push_head(zb - M + t);
+        push_head(zb - M + t, t);
     }
     load_plane(zb);
-    store_plane(lds[zb & 1]);
+    store_plane(lds[zb & 1], M);
     __syncthreads();
 
-    for (int z = zb; z < ze; ++z) {
+    // rounds of Q planes with compile-time queue slots: at step S, plane z + d
+    // sits in slot (S + M + d) mod Q; the slot of plane z - M takes plane z + M + 1
+    for (int z0 = zb; z0 < ze; z0 += Q) {
+        static_for<0, Q>([&](auto Sc) -> bool {
+        constexpr int S = decltype(Sc)::value;
+        constexpr int C = (S + M) % Q;
+        const int z = z0 + S;
+        if (z >= ze) return false;
         const float *P = lds[z & 1];
         float clp[RY], czx[RY], cpx[RY], czy[RY], cpy[RY], czz[RY], cpz[RY];
 #pragma unroll
@@ -649,7 +686,7 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
             load_head(zn + M);
             load_plane(zn);
         }
-        const int64_t pz = int64_t(z) * sxy;
+        const uint32_t so = pl(z);
         float az = 0.f, bz = 0.f, daz = 0.f;
         if constexpr (BZ) {
             az = pr.a[2][z];
@@ -661,12 +698,13 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
             if (!L.act[r]) continue;
             const int row = (w + 4 * r + M) * LW + lane + M;
             const float *Gw = P + row;
-            const int64_t i = pz + col[r];
-            const float gun = ql[r][0];
-            const float gw = qg[r][M];
+            const float gun = ql[r][C];
+            const float gw = qg[r][C];
             if (grad && img && L.x >= g.X0 && L.x < g.X1 && L.y[r] >= g.Y0 && L.y[r] < g.Y1 && z >= g.Z0 &&
-                z < g.Z1)
-                grad[off + i] += 2.f * g.dt * g.dt * img[off + i] * gun / vp[i];
+                z < g.Z1) {
+                const int64_t i = off + int64_t(z) * sxy + col[r] / 4u;
+                grad[i] += 2.f * g.dt * g.dt * img[i] * gun / vp.ld(col[r], so);
+            }
             float out;
             if constexpr (!SHELL) {
                 float lx = 0.f, ly = 0.f, lz = 0.f;
@@ -674,7 +712,7 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                 for (int t = 1; t <= M; ++t) {
                     lx += c.lap[t] * ((Gw[t] - gw) + (Gw[-t] - gw));
                     ly += c.lap[t] * ((Gw[t * LW] - gw) + (Gw[-t * LW] - gw));
-                    lz += c.lap[t] * ((qg[r][M + t] - gw) + (qg[r][M - t] - gw));
+                    lz += c.lap[t] * ((qg[r][(C + t) % Q] - gw) + (qg[r][(C - t + Q) % Q] - gw));
                 }
                 out = 2.0f * gun - clp[r] + lx * g.inv_dx2 + ly * g.inv_dy2 + lz * g.inv_dz2;
             } else {
@@ -694,8 +732,8 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                     Lt[0] = Ls * g.inv_dx2;
                     Dg[0] = Dgg * g.inv_dx;
                     Dq *= g.inv_dx;
-                    f.psio[0][off + i] = ax * cpx[r] + dax * gt0 - Dq;
-                    f.zetao[0][off + i] = ax * (czx[r] + gw);
+                    pox.st(ax * cpx[r] + dax * gt0 - Dq, col[r], so);
+                    zox.st(ax * (czx[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
@@ -718,8 +756,8 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                     Lt[1] = Ls * g.inv_dy2;
                     Dg[1] = Dgg * g.inv_dy;
                     Dq *= g.inv_dy;
-                    f.psio[1][off + i] = ay[r] * cpy[r] + day[r] * gt0 - Dq;
-                    f.zetao[1][off + i] = ay[r] * (czy[r] + gw);
+                    poy.st(ay[r] * cpy[r] + day[r] * gt0 - Dq, col[r], so);
+                    zoy.st(ay[r] * (czy[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
@@ -734,29 +772,32 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                     float Ls = 0.f, Dgg = 0.f, Dq = 0.f;
 #pragma unroll
                     for (int t = 1; t <= M; ++t) {
-                        Ls += c.lap[t] * ((qu[r][M + t] - l0) + (qu[r][M - t] - l0));
-                        Dgg += c.grad[t] * (qv[r][M + t] - qv[r][M - t]);
-                        Dq += c.grad[t] * (qw[r][M + t] - qw[r][M - t]);
+                        Ls += c.lap[t] * ((qu[r][(C + t) % Q] - l0) + (qu[r][(C - t + Q) % Q] - l0));
+                        Dgg += c.grad[t] * (qv[r][(C + t) % Q] - qv[r][(C - t + Q) % Q]);
+                        Dq += c.grad[t] * (qw[r][(C + t) % Q] - qw[r][(C - t + Q) % Q]);
                     }
                     Lt[2] = Ls * g.inv_dz2;
                     Dg[2] = Dgg * g.inv_dz;
                     Dq *= g.inv_dz;
-                    f.psio[2][off + i] = az * cpz[r] + daz * gt0 - Dq;
-                    f.zetao[2][off + i] = az * (czz[r] + gw);
+                    poz.st(az * cpz[r] + daz * gt0 - Dq, col[r], so);
+                    zoz.st(az * (czz[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
-                    for (int t = 1; t <= M; ++t) Ls += c.lap[t] * ((qg[r][M + t] - gw) + (qg[r][M - t] - gw));
+                    for (int t = 1; t <= M; ++t)
+                        Ls += c.lap[t] * ((qg[r][(C + t) % Q] - gw) + (qg[r][(C - t + Q) % Q] - gw));
                     Lt[2] = Ls * g.inv_dz2;
                     Dg[2] = 0.f;
                 }
                 out = 2.0f * gun - clp[r] + (Lt[0] + Lt[1] + Lt[2] - Dg[0] - Dg[1] - Dg[2]);
             }
-            lnx[i] = out;
+            lnx.st(out, col[r], so);
         }
-        push_head(min(z + 1, ze - 1) + M);
-        if (z + 1 < ze) store_plane(lds[(z + 1) & 1]);
+        push_head(min(z + 1, ze - 1) + M, S);
+        if (z + 1 < ze) store_plane(lds[(z + 1) & 1], (C + 1) % Q);
         __syncthreads();
+        return true;
+        });
     }
 }
 
@@ -771,6 +812,7 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
 {
     constexpr int TY = 4 * RY;
     using PM = PlaneMap<M, TY>;
+    constexpr int LW = PM::LW;
     __shared__ float lds[2][PM::LH * PM::LW];
     int b, k, xo, yo, zb, ze;
     if (!zcoords<TY>(zs, b, k, xo, yo, zb, ze)) return;
@@ -781,50 +823,73 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
     const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
     const int64_t off = int64_t(b) * g.n;
-    const float *__restrict__ un = u_now + off;
-    const float *__restrict__ up = u_prev + off;
-    float *__restrict__ unx = u_next + off;
-    const float *__restrict__ vp = vpb + int64_t(b) * g.mstride;
     const float dt = g.dt;
+    const int zbase = zb - M;
+    const int64_t fb = off + int64_t(zbase) * sxy, fe = off + g.n;
+    const int64_t mb = int64_t(b) * g.mstride + int64_t(zbase) * sxy, me = int64_t(b) * g.mstride + g.n;
+    const uint32_t sxy4 = uint32_t(sxy) * 4u;
+    auto pl = [&](int z) { return uint32_t(z - zbase) * sxy4; };
+    const Fld un(u_now, fb, fe), up(u_prev, fb, fe), unx(u_next, fb, fe), vp(vpb, mb, me);
+    const Fld ad(adj, fb, fe), gr(grad, fb, fe);   // unused without grad
     const bool img_x = L.x >= g.X0 + g.W && L.x < g.X1 - g.W;
     bool img_y[RY];
-    int64_t col[RY], xh_row[RY];
+    uint32_t col[RY], xhc[RY], yhc[PM::YH];   // in-plane byte offsets
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-        col[r] = int64_t(L.yc[r]) * nx + L.xc;
-        xh_row[r] = int64_t(L.yc[r]) * nx + pm.xh_col;
+        col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
+        xhc[r] = uint32_t(L.yc[r] * nx + pm.xh_col) * 4u;
         img_y[r] = L.y[r] >= g.Y0 + g.W && L.y[r] < g.Y1 - g.W;
     }
-    float q[RY][2 * M + 1];
+#pragma unroll
+    for (int t = 0; t < PM::YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
+
+    constexpr int Q = 2 * M + 1;
+    float q[RY][Q];
     float nq[RY], nup[RY], nv[RY], nxh[RY], na[RY], ng[RY];
     float nyh[PM::YH];
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-#pragma unroll
-        for (int t = 0; t <= 2 * M; ++t) q[r][t] = un[int64_t(zb - M + t) * sxy + col[r]];
-        nup[r] = up[int64_t(zb) * sxy + col[r]];
-        nv[r] = vp[int64_t(zb) * sxy + col[r]];
-        na[r] = grad ? adj[off + int64_t(zb) * sxy + col[r]] : 0.f;
-        ng[r] = grad ? grad[off + int64_t(zb) * sxy + col[r]] : 0.f;
-        nxh[r] = un[int64_t(zb) * sxy + xh_row[r]];
-    }
-#pragma unroll
-    for (int t = 0; t < PM::YH; ++t)
-        nyh[t] = pm.yh_row[t] >= 0 ? un[int64_t(zb) * sxy + int64_t(pm.yh_row[t]) * nx + L.xc] : 0.f;
-    {
-        float *P = lds[zb & 1];
+    auto load_plane = [&](int zn) {
+        const uint32_t so = pl(zn);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            P[(w + 4 * r + M) * PM::LW + lane + M] = q[r][M];
-            if (pm.xh_slot >= 0) P[(w + 4 * r + M) * PM::LW + pm.xh_slot] = nxh[r];
+            nup[r] = up.ld(col[r], so);
+            nv[r] = vp.ld(col[r], so);
+            if (grad) {
+                na[r] = ad.ld(col[r], so);
+                ng[r] = gr.ld(col[r], so);
+            }
+            nxh[r] = un.ld(xhc[r], so);
         }
 #pragma unroll
         for (int t = 0; t < PM::YH; ++t)
-            if (pm.yh_row[t] >= 0) P[pm.yh_slot[t] * PM::LW + lane + M] = nyh[t];
+            if (pm.yh_row[t] >= 0) nyh[t] = un.ld(yhc[t], so);
+    };
+    auto store_plane = [&](float *P, int cs) {
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            P[(w + 4 * r + M) * LW + lane + M] = q[r][cs];
+            if (pm.xh_slot >= 0) P[(w + 4 * r + M) * LW + pm.xh_slot] = nxh[r];
+        }
+#pragma unroll
+        for (int t = 0; t < PM::YH; ++t)
+            if (pm.yh_row[t] >= 0) P[pm.yh_slot[t] * LW + lane + M] = nyh[t];
+    };
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        na[r] = ng[r] = 0.f;
+#pragma unroll
+        for (int t = 0; t < Q; ++t) q[r][t] = un.ld(col[r], pl(zb - M + t));
     }
+    load_plane(zb);
+    store_plane(lds[zb & 1], M);
     __syncthreads();
 
-    for (int z = zb; z < ze; ++z) {
+    // rounds of Q planes with compile-time queue slots (as in fwd3_kernel)
+    for (int z0 = zb; z0 < ze; z0 += Q) {
+        static_for<0, Q>([&](auto Sc) -> bool {
+        constexpr int S = decltype(Sc)::value;
+        constexpr int C = (S + M) % Q;
+        const int z = z0 + S;
+        if (z >= ze) return false;
         const float *P = lds[z & 1];
         float cup[RY], cv[RY], ca[RY], cg[RY];
 #pragma unroll
@@ -836,59 +901,35 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
         }
         {
             const int zn = min(z + 1, ze - 1);
-            const int64_t pz = int64_t(zn) * sxy;
+            const uint32_t so = pl(zn + M);
 #pragma unroll
-            for (int r = 0; r < RY; ++r) {
-                nq[r] = un[pz + int64_t(M) * sxy + col[r]];
-                nup[r] = up[pz + col[r]];
-                nv[r] = vp[pz + col[r]];
-                if (grad) {
-                    na[r] = adj[off + pz + col[r]];
-                    ng[r] = grad[off + pz + col[r]];
-                }
-                nxh[r] = un[pz + xh_row[r]];
-            }
-#pragma unroll
-            for (int t = 0; t < PM::YH; ++t)
-                if (pm.yh_row[t] >= 0) nyh[t] = un[pz + int64_t(pm.yh_row[t]) * nx + L.xc];
+            for (int r = 0; r < RY; ++r) nq[r] = un.ld(col[r], so);
+            load_plane(zn);
         }
-        const int64_t pz = int64_t(z) * sxy;
+        const uint32_t so = pl(z);
         const bool img_z = z >= g.Z0 + g.W && z < g.Z1 - g.W;
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             if (!L.act[r]) continue;
-            const float *Rw = P + (w + 4 * r + M) * PM::LW + lane + M;
-            const float u0 = q[r][M];
+            const float *Rw = P + (w + 4 * r + M) * LW + lane + M;
+            const float u0 = q[r][C];
             const float w_sum = lap_taps<M>(c, [&](int t) { return Rw[t]; }, g.inv_dx2) +
-                                lap_taps<M>(c, [&](int t) { return Rw[t * PM::LW]; }, g.inv_dy2) +
-                                lap_taps<M>(c, [&](int t) { return q[r][M + t]; }, g.inv_dz2);
+                                lap_taps<M>(c, [&](int t) { return Rw[t * LW]; }, g.inv_dy2) +
+                                lap_taps<M>(c, [&](int t) { return q[r][(C + t + Q) % Q]; }, g.inv_dz2);
             const float v = cv[r];
             const float unxt = 2.0f * u0 - cup[r] + (v * v) * dt * dt * w_sum;
-            const int64_t i = pz + col[r];
-            unx[i] = unxt;
+            unx.st(unxt, col[r], so);
             if (grad && img_x && img_y[r] && img_z) {
                 const float u_tt = (unxt - 2 * u0 + cup[r]) * g.inv_dt2;
-                grad[off + i] = cg[r] + 2.f * dt * dt * u_tt * ca[r] / v;
+                gr.st(cg[r] + 2.f * dt * dt * u_tt * ca[r] / v, col[r], so);
             }
         }
 #pragma unroll
-        for (int r = 0; r < RY; ++r) {
-#pragma unroll
-            for (int t = 0; t < 2 * M; ++t) q[r][t] = q[r][t + 1];
-            q[r][2 * M] = nq[r];
-        }
-        if (z + 1 < ze) {
-            float *Pn = lds[(z + 1) & 1];
-#pragma unroll
-            for (int r = 0; r < RY; ++r) {
-                Pn[(w + 4 * r + M) * PM::LW + lane + M] = q[r][M];
-                if (pm.xh_slot >= 0) Pn[(w + 4 * r + M) * PM::LW + pm.xh_slot] = nxh[r];
-            }
-#pragma unroll
-            for (int t = 0; t < PM::YH; ++t)
-                if (pm.yh_row[t] >= 0) Pn[pm.yh_slot[t] * PM::LW + lane + M] = nyh[t];
-        }
+        for (int r = 0; r < RY; ++r) q[r][S] = nq[r];
+        if (z + 1 < ze) store_plane(lds[(z + 1) & 1], (C + 1) % Q);
         __syncthreads();
+        return true;
+        });
     }
 }
 
@@ -1009,7 +1050,15 @@ void adj_region(const Ctx &x, const AdjFields &f, const float *img, float *grad)
 
 }  // namespace
 
-bool zm_supported(const Geo &g) { return g.M >= 1 && g.M <= 4 && g.sxy < (int64_t(1) << 31); }
+bool zm_supported(const Geo &g, int &tz)
+{
+    if (g.M < 1 || g.M > 4) return false;
+    const int64_t planes = ((int64_t(1) << 32) - 1) / (4 * g.sxy);   // planes addressable from a segment base
+    const int64_t fit = planes - 2 * g.M - 1;
+    if (fit < 1) return false;
+    tz = int(std::min<int64_t>(tz, fit));
+    return true;
+}
 
 int shell_boxes(const Geo &g, Box out[6])
 {
