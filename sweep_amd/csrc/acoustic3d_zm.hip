@@ -39,8 +39,12 @@ constexpr int kRectMax = 8;
 
 // A box of cells [x0, x1) x [y0, y1) x [z0, z1) cut into tiles of 64 x TY
 // columns (partial tiles masked) and z segments of tz planes.
+// Tiles start at xs <= x0 (x0 rounded down to a 128-byte line when rows are
+// line-aligned), so each wave row covers whole lines; lanes outside [x0, x1)
+// are masked.
 struct ZRect {
     int x0, x1, y0, y1, z0, z1, tz;
+    int xs;
     int ntx, nty, nseg, nblk;
 };
 
@@ -89,6 +93,16 @@ struct Fld {
     {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
     }
+    // Streaming forms (non-temporal cache policy) for data no other lane of
+    // the step reads again, so it does not push the shared halo planes out of L2.
+    __device__ __forceinline__ float ld_nt(uint32_t voff, uint32_t soff) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+    }
+    __device__ __forceinline__ void st_nt(float v, uint32_t voff, uint32_t soff) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 2);
+    }
 };
 
 // This block's shot, tile origin and z segment; false for a block with no
@@ -106,7 +120,7 @@ __device__ __forceinline__ bool zcoords(const ZSet &s, int &b, int &k, int &xo, 
             const int tx = t % R.ntx;
             const int ty = (t / R.ntx) % R.nty;
             const int sg = t / (R.ntx * R.nty);
-            xo = R.x0 + 64 * tx;
+            xo = R.xs + 64 * tx;
             yo = R.y0 + TY * ty;
             zb = R.z0 + sg * R.tz;
             ze = min(zb + R.tz, R.z1);
@@ -165,13 +179,38 @@ struct Lanes {
     __device__ Lanes(const ZRect &R, int lane, int w, int xo, int yo, int nx, int ny)
     {
         x = xo + lane;
-        xa = x < R.x1;
+        xa = x >= R.x0 && x < R.x1;
         xc = min(x, nx - 1);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             y[r] = yo + w + 4 * r;
             yc[r] = min(y[r], ny - 1);
             act[r] = xa && y[r] < R.y1;
+        }
+    }
+};
+
+// ---------------------------------------------------------------- halo duties
+// The x-halo of a tile plane is RY rows x 2M columns per wave; lane l < 2M RY
+// of wave w loads row w + 4 (l / 2M), column M - ... of it, so each lane holds
+// at most one x-halo value per plane.  The y-halo rows are spread as in
+// PlaneMap (rows j = w + 4t of the 2M rows above and below the tile).
+template <int M, int RY>
+struct XHalo {
+    uint32_t off;   // in-plane byte offset (clamped into the grid)
+    int lds;        // index in the LDS plane, -1: no duty
+    __device__ XHalo(int lane, int w, int xo, int yo, int nx, int ny)
+    {
+        if (lane < 2 * M * RY) {
+            const int r = lane / (2 * M), c = lane % (2 * M);
+            const int row = w + 4 * r;
+            const int gx = c < M ? xo - M + c : xo + 64 + (c - M);
+            const int gy = min(yo + row, ny - 1);
+            off = uint32_t(gy * nx + min(max(gx, 0), nx - 1)) * 4u;
+            lds = (row + M) * (64 + 2 * M) + (c < M ? c : 64 + M + (c - M));
+        } else {
+            off = 0;
+            lds = -1;
         }
     }
 };
@@ -185,25 +224,35 @@ struct Lanes {
 // product-rule update on the axes whose band holds it (pos < lo + 1 or
 // pos >= hi - 1), the plain second derivative on the others, and saves its
 // strip cells.
+//
+// The z loop runs in rounds of R = 2M + 2 planes with every queue slot fixed
+// at compile time.  At step S of a round (plane z), the u queue slot
+// (S + M + d) mod R holds plane z + d for |d| <= M, and the free slot takes
+// the head z + M + 1.  The x/y halo of u is fetched together with the head
+// (when the neighbouring tiles fetch that plane as their own head, so the
+// shared lines are still in L2) and waits in an (M + 1)-deep queue until the
+// plane is staged in LDS; halo slot p mod (M + 1) holds plane zb + p.
 template <int M, int RY, bool BX, bool BY, bool BZ>
-__global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepFields f, float *__restrict__ u_this,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) fwd3_kernel(Geo g, Coef c, Prof pr, StepFields f, float *__restrict__ u_this,
                                                    float *__restrict__ strip, ZSet zs)
 {
     constexpr bool SHELL = BX || BY || BZ;
     constexpr int TY = 4 * RY;
     using PM = PlaneMap<M, TY>;
-    constexpr int LW = PM::LW, PS = PM::LH * PM::LW;
+    constexpr int LW = PM::LW, PS = PM::LH * PM::LW, YH = PM::YH;
     constexpr int NPL = 1 + (BX ? 1 : 0) + (BY ? 1 : 0);
     constexpr int SX = 1, SY = BX ? 2 : 1;   // plane slots of psi_x, psi_y
+    constexpr int R = 2 * M + 2, H = M + 1;
     __shared__ float lds[2][NPL * PS];
     int b, k, xo, yo, zb, ze;
     if (!zcoords<TY>(zs, b, k, xo, yo, zb, ze)) return;
-    const ZRect R = zs.r[k];
+    const ZRect Rc = zs.r[k];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nx = g.nx, ny = g.ny;
     const int64_t sxy = g.sxy;
-    const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
+    const Lanes<M, RY> L(Rc, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
+    const XHalo<M, RY> xh(lane, w, xo, yo, nx, ny);
     const int64_t off = int64_t(b) * g.n;
     const float dt = g.dt;
     // fields from plane zb - M on; plane z sits at byte offset pl(z)
@@ -213,15 +262,15 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
     const uint32_t sxy4 = uint32_t(sxy) * 4u;
     auto pl = [&](int z) { return uint32_t(z - zbase) * sxy4; };
     const Fld un(f.u_now, fb, fe), up(f.u_prev, fb, fe), unx(f.u_next, fb, fe), vp(f.vp, mb, me);
+    const Fld psx(f.psi[0], fb, fe), psy(f.psi[1], fb, fe), psz(f.psi[2], fb, fe);
+    const Fld pnx(f.psin[0], fb, fe), pny(f.psin[1], fb, fe), pnz(f.psin[2], fb, fe);
+    const Fld zex(f.zeta[0], fb, fe), zey(f.zeta[1], fb, fe), zez(f.zeta[2], fb, fe);
 
-    uint32_t col[RY], xhc[RY], yhc[PM::YH];   // in-plane byte offsets
+    uint32_t col[RY], yhc[YH];   // in-plane byte offsets
 #pragma unroll
-    for (int r = 0; r < RY; ++r) {
-        col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
-        xhc[r] = uint32_t(L.yc[r] * nx + pm.xh_col) * 4u;
-    }
+    for (int r = 0; r < RY; ++r) col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
 #pragma unroll
-    for (int t = 0; t < PM::YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
+    for (int t = 0; t < YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
 
     float ax = 0.f, bx = 0.f, dbx = 0.f, dax = 0.f;
     bool inbx = false;
@@ -246,63 +295,68 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
             day[r] = d1_taps<M>(c, [&](int q) { return pr.a[1][ys + q]; }, g.inv_dy);
         }
     }
-    const Fld psx(f.psi[0], fb, fe), psy(f.psi[1], fb, fe), psz(f.psi[2], fb, fe);
-    const Fld pnx(f.psin[0], fb, fe), pny(f.psin[1], fb, fe), pnz(f.psin[2], fb, fe);
-    const Fld zex(f.zeta[0], fb, fe), zey(f.zeta[1], fb, fe), zez(f.zeta[2], fb, fe);
 
-    float q[RY][2 * M + 1], qp[RY][BZ ? 2 * M + 1 : 1];
-    // plane-(z+1) registers
-    float nq[RY], nqp[RY], nup[RY], nv[RY], nxh[RY], nyh[PM::YH];
-    float npx[RY], npxh[RY], nzx[RY], npy[RY], npyh[PM::YH], nzy[RY], nzz[RY];
+    float q[RY][R], qp[RY][BZ ? R : 1];   // u and psi_z z-queues
+    float hx[H], hy[H][YH];               // u halo queue
+    // plane-(z+1) registers (single-use fields, fetched one plane ahead)
+    float nup[RY], nv[RY], npx[RY], nzx[RY], npy[RY], nzy[RY], nzz[RY];
+    float npxh = 0.f, npyh[YH];           // psi_x x-halo, psi_y y-halo
 
-    auto load_plane = [&](int zn) {   // everything plane zn needs except the queue heads
+    auto load_halo = [&](int zh, int slot) {   // the u halo of plane zh into halo slot `slot`
+        const uint32_t so = pl(zh);
+        if (xh.lds >= 0) hx[slot] = un.ld(xh.off, so);
+#pragma unroll
+        for (int t = 0; t < YH; ++t)
+            if (pm.yh_row[t] >= 0) hy[slot][t] = un.ld(yhc[t], so);
+    };
+    auto load_plane = [&](int zn) {   // the single-use fields of plane zn
         const uint32_t so = pl(zn);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            nup[r] = up.ld(col[r], so);
-            nv[r] = vp.ld(col[r], so);
-            nxh[r] = un.ld(xhc[r], so);
+            nup[r] = up.ld_nt(col[r], so);
+            nv[r] = vp.ld_nt(col[r], so);
             if constexpr (BX) {
                 npx[r] = psx.ld(col[r], so);
-                npxh[r] = psx.ld(xhc[r], so);
-                nzx[r] = zex.ld(col[r], so);
+                nzx[r] = zex.ld_nt(col[r], so);
             }
             if constexpr (BY) {
                 npy[r] = psy.ld(col[r], so);
-                nzy[r] = zey.ld(col[r], so);
+                nzy[r] = zey.ld_nt(col[r], so);
             }
-            if constexpr (BZ) nzz[r] = zez.ld(col[r], so);
+            if constexpr (BZ) nzz[r] = zez.ld_nt(col[r], so);
         }
+        if constexpr (BX)
+            if (xh.lds >= 0) npxh = psx.ld(xh.off, so);
+        if constexpr (BY) {
 #pragma unroll
-        for (int t = 0; t < PM::YH; ++t) {
-            if (pm.yh_row[t] >= 0) {
-                nyh[t] = un.ld(yhc[t], so);
-                if constexpr (BY) npyh[t] = psy.ld(yhc[t], so);
-            }
+            for (int t = 0; t < YH; ++t)
+                if (pm.yh_row[t] >= 0) npyh[t] = psy.ld(yhc[t], so);
         }
     };
-    auto store_plane = [&](float *P, int cs) {   // stage the plane whose centre sits in queue slot cs
+    // stage plane z: its u from queue slot cs, its u halo from halo slot hs
+    auto store_plane = [&](float *P, int cs, int hs) {
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             const int row = (w + 4 * r + M) * LW;
             P[row + lane + M] = q[r][cs];
-            if (pm.xh_slot >= 0) P[row + pm.xh_slot] = nxh[r];
-            if constexpr (BX) {
-                P[SX * PS + row + lane + M] = npx[r];
-                if (pm.xh_slot >= 0) P[SX * PS + row + pm.xh_slot] = npxh[r];
-            }
+            if constexpr (BX) P[SX * PS + row + lane + M] = npx[r];
             if constexpr (BY) P[SY * PS + row + lane + M] = npy[r];
         }
+        if (xh.lds >= 0) {
+            P[xh.lds] = hx[hs];
+            if constexpr (BX) P[SX * PS + xh.lds] = npxh;
+        }
 #pragma unroll
-        for (int t = 0; t < PM::YH; ++t) {
+        for (int t = 0; t < YH; ++t) {
             if (pm.yh_row[t] >= 0) {
-                P[pm.yh_slot[t] * LW + lane + M] = nyh[t];
+                P[pm.yh_slot[t] * LW + lane + M] = hy[hs][t];
                 if constexpr (BY) P[SY * PS + pm.yh_slot[t] * LW + lane + M] = npyh[t];
             }
         }
     };
 
-    // prologue: queue slots t = 0..2M hold planes zb - M + t, plane zb staged
+    // prologue: u slots t = 0..2M hold planes zb - M + t, halo slots p hold
+    // planes zb + p (p <= M), plane zb staged in buffer 0
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
 #pragma unroll
@@ -311,21 +365,19 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
             if constexpr (BZ) qp[r][t] = psz.ld(col[r], pl(zb - M + t));
         }
     }
+#pragma unroll
+    for (int p = 0; p < H; ++p) load_halo(min(zb + p, ze - 1 + M), p);
     load_plane(zb);
-    store_plane(lds[zb & 1], M);
+    store_plane(lds[0], M, 0);
     __syncthreads();
 
-    // The z loop runs in rounds of Q = 2M+1 planes with the queue rotation
-    // resolved at compile time: at step S of a round, plane z + d sits in slot
-    // (S + M + d) mod Q, and the slot of plane z - M takes plane z + M + 1.
-    constexpr int Q = 2 * M + 1;
-    for (int z0 = zb; z0 < ze; z0 += Q) {
-        static_for<0, Q>([&](auto Sc) -> bool {
+    for (int z0 = zb; z0 < ze; z0 += R) {
+        static_for<0, R>([&](auto Sc) -> bool {
         constexpr int S = decltype(Sc)::value;
-        constexpr int C = (S + M) % Q;
+        constexpr int C = (S + M) % R;       // slot of plane z
         const int z = z0 + S;
         if (z >= ze) return false;
-        const float *P = lds[z & 1];
+        const float *P = lds[S & 1];
         float cup[RY], cv[RY], czx[RY], czy[RY], czz[RY], cpx[RY], cpy[RY];
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
@@ -337,15 +389,17 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
             cpx[r] = npx[r];
             cpy[r] = npy[r];
         }
-        {   // issue plane z+1 (clamped on the last plane; unused there)
-            const int zn = min(z + 1, ze - 1);
-            const uint32_t so = pl(zn + M);
+        {   // plane z + M + 1: u head and u halo; plane z + 1: the rest
+            // (clamped at the segment end, where they are not used)
+            const int zh = min(z + M + 1, ze - 1 + M);
+            const uint32_t so = pl(zh);
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
-                nq[r] = un.ld(col[r], so);
-                if constexpr (BZ) nqp[r] = psz.ld(col[r], so);
+                q[r][(S + 2 * M + 1) % R] = un.ld(col[r], so);
+                if constexpr (BZ) qp[r][(S + 2 * M + 1) % R] = psz.ld(col[r], so);
             }
-            load_plane(zn);
+            load_halo(zh, S % H);
+            load_plane(min(z + 1, ze - 1));
         }
         const uint32_t so = pl(z);
         bool inbz = false;
@@ -363,9 +417,10 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
             const int row = (w + 4 * r + M) * LW + lane + M;
             const float *Rw = P + row;
             const float u0 = q[r][C];
+            auto qz = [&](int t) { return q[r][(C + t + R) % R]; };
             const float lx = lap_taps<M>(c, [&](int t) { return Rw[t]; }, g.inv_dx2);
             const float ly = lap_taps<M>(c, [&](int t) { return Rw[t * LW]; }, g.inv_dy2);
-            const float lz = lap_taps<M>(c, [&](int t) { return q[r][(C + t + Q) % Q]; }, g.inv_dz2);
+            const float lz = lap_taps<M>(c, qz, g.inv_dz2);
             float w_sum;
             if constexpr (!SHELL) {
                 w_sum = lx + ly + lz;
@@ -382,10 +437,10 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
                         const float dpsi = d1_taps<M>(c, [&](int t) { return Px[t]; }, g.inv_dx);
                         const float psi0 = cpx[r];
                         const float tmp = ((1.0f + bx) * lx + dbx * du) + ax * dpsi + dax * psi0;
-                        const float z0 = czx[r];
-                        w_sum += (1.0f + bx) * tmp + ax * z0;
-                        pnx.st(bx * du + ax * psi0, col[r], so);
-                        zex.st(bx * tmp + ax * z0, col[r], so);
+                        const float z0v = czx[r];
+                        w_sum += (1.0f + bx) * tmp + ax * z0v;
+                        pnx.st_nt(bx * du + ax * psi0, col[r], so);
+                        zex.st_nt(bx * tmp + ax * z0v, col[r], so);
                     } else {
                         w_sum += lx;
                     }
@@ -395,23 +450,24 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
                         const float dpsi = d1_taps<M>(c, [&](int t) { return Py[t * LW]; }, g.inv_dy);
                         const float psi0 = cpy[r];
                         const float tmp = ((1.0f + by[r]) * ly + dby[r] * du) + ay[r] * dpsi + day[r] * psi0;
-                        const float z0 = czy[r];
-                        w_sum += (1.0f + by[r]) * tmp + ay[r] * z0;
-                        pny.st(by[r] * du + ay[r] * psi0, col[r], so);
-                        zey.st(by[r] * tmp + ay[r] * z0, col[r], so);
+                        const float z0v = czy[r];
+                        w_sum += (1.0f + by[r]) * tmp + ay[r] * z0v;
+                        pny.st_nt(by[r] * du + ay[r] * psi0, col[r], so);
+                        zey.st_nt(by[r] * tmp + ay[r] * z0v, col[r], so);
                     } else {
                         w_sum += ly;
                     }
                     if constexpr (BZ) {
                         if (bandz) {
-                            const float du = d1_taps<M>(c, [&](int t) { return q[r][(C + t + Q) % Q]; }, g.inv_dz);
-                            const float dpsi = d1_taps<M>(c, [&](int t) { return qp[r][(C + t + Q) % Q]; }, g.inv_dz);
+                            auto qpz = [&](int t) { return qp[r][(C + t + R) % R]; };
+                            const float du = d1_taps<M>(c, qz, g.inv_dz);
+                            const float dpsi = d1_taps<M>(c, qpz, g.inv_dz);
                             const float psi0 = qp[r][C];
                             const float tmp = ((1.0f + bz) * lz + dbz * du) + az * dpsi + daz * psi0;
-                            const float z0 = czz[r];
-                            w_sum += (1.0f + bz) * tmp + az * z0;
-                            pnz.st(bz * du + az * psi0, col[r], so);
-                            zez.st(bz * tmp + az * z0, col[r], so);
+                            const float z0v = czz[r];
+                            w_sum += (1.0f + bz) * tmp + az * z0v;
+                            pnz.st_nt(bz * du + az * psi0, col[r], so);
+                            zez.st_nt(bz * tmp + az * z0v, col[r], so);
                         } else {
                             w_sum += lz;
                         }
@@ -421,18 +477,13 @@ __global__ void __launch_bounds__(256) fwd3_kernel(Geo g, Coef c, Prof pr, StepF
                 }
             }
             const float v = cv[r];
-            unx.st(2.0f * u0 - cup[r] + (v * v) * dt * dt * w_sum, col[r], so);
+            unx.st_nt(2.0f * u0 - cup[r] + (v * v) * dt * dt * w_sum, col[r], so);
             if (u_this) u_this[off + int64_t(z) * sxy + col[r] / 4u] = (v * v) * w_sum;
         }
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {   // plane z-M leaves slot S, plane z+M+1 enters it
-            q[r][S] = nq[r];
-            if constexpr (BZ) qp[r][S] = nqp[r];
-        }
-        if (z + 1 < ze) store_plane(lds[(z + 1) & 1], (C + 1) % Q);
+        if (z + 1 < ze) store_plane(lds[(S + 1) & 1], (C + 1) % R, (S + 1) % H);
         __syncthreads();
         return true;
-            });
+        });
     }
 }
 
@@ -936,22 +987,23 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
 // ---------------------------------------------------------------- host plans
 constexpr int kRyInt = 4;     // rows per lane of the interior kernels (tile 64 x 16)
 
-ZRect make_rect(int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY)
+ZRect make_rect(int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, int nx)
 {
     ZRect R{};
     R.x0 = x0; R.x1 = x1; R.y0 = y0; R.y1 = y1; R.z0 = z0; R.z1 = z1; R.tz = tz;
-    R.ntx = (x1 - x0 + 63) / 64;
+    R.xs = nx % 32 == 0 ? x0 - x0 % 32 : x0;
+    R.ntx = (x1 - R.xs + 63) / 64;
     R.nty = (y1 - y0 + TY - 1) / TY;
     R.nseg = (z1 - z0 + tz - 1) / tz;
     R.nblk = R.ntx * R.nty * R.nseg;
     return R;
 }
 
-void add_rect(ZSet &s, int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY)
+void add_rect(ZSet &s, int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, int nx)
 {
     if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
     SWP_CHECK(s.n < kRectMax, "acoustic3d: too many block rectangles");
-    s.r[s.n] = make_rect(x0, x1, y0, y1, z0, z1, tz, TY);
+    s.r[s.n] = make_rect(x0, x1, y0, y1, z0, z1, tz, TY, nx);
     s.per_shot += s.r[s.n].nblk;
     ++s.n;
 }
@@ -997,7 +1049,7 @@ ZSet region_set(const Geo &g, bool ex, bool ey, bool ez, int tz, int TY)
     for (int i = 0; i < nzr; ++i)
         for (int j = 0; j < nyr; ++j)
             for (int l = 0; l < nxr; ++l)
-                add_rect(s, xs[l][0], xs[l][1], ys[j][0], ys[j][1], zs[i][0], zs[i][1], tz, TY);
+                add_rect(s, xs[l][0], xs[l][1], ys[j][0], ys[j][1], zs[i][0], zs[i][1], tz, TY, g.nx);
     return s;
 }
 
@@ -1116,7 +1168,7 @@ void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const f
 {
     const Geo &g = x.g;
     ZSet s{};
-    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyInt);
+    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyInt, x.g.nx);
     if (!s.per_shot) return;
     by_order(g.M, [&](auto ms) {
         constexpr int M = decltype(ms)::value;
