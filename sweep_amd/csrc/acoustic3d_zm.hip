@@ -97,11 +97,11 @@ struct Fld {
     // the step reads again, so it does not push the shared halo planes out of L2.
     __device__ __forceinline__ float ld_nt(uint32_t voff, uint32_t soff) const
     {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
     }
     __device__ __forceinline__ void st_nt(float v, uint32_t voff, uint32_t soff) const
     {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 2);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
     }
 };
 
@@ -192,12 +192,14 @@ struct Lanes {
 
 // ---------------------------------------------------------------- halo duties
 // The x-halo of a tile plane is RY rows x 2M columns per wave; lane l < 2M RY
-// of wave w loads row w + 4 (l / 2M), column M - ... of it, so each lane holds
-// at most one x-halo value per plane.  The y-halo rows are spread as in
+// of wave w takes row w + 4 (l / 2M) and halo column l mod 2M of it (M on the
+// left of the tile, then M on the right), so each lane holds at most one
+// x-halo value per plane.  The y-halo rows are spread as in
 // PlaneMap (rows j = w + 4t of the 2M rows above and below the tile).
 template <int M, int RY>
 struct XHalo {
     uint32_t off;   // in-plane byte offset (clamped into the grid)
+    int col;        // global column (clamped)
     int lds;        // index in the LDS plane, -1: no duty
     __device__ XHalo(int lane, int w, int xo, int yo, int nx, int ny)
     {
@@ -206,9 +208,11 @@ struct XHalo {
             const int row = w + 4 * r;
             const int gx = c < M ? xo - M + c : xo + 64 + (c - M);
             const int gy = min(yo + row, ny - 1);
-            off = uint32_t(gy * nx + min(max(gx, 0), nx - 1)) * 4u;
+            col = min(max(gx, 0), nx - 1);
+            off = uint32_t(gy * nx + col) * 4u;
             lds = (row + M) * (64 + 2 * M) + (c < M ? c : 64 + M + (c - M));
         } else {
+            col = 0;
             off = 0;
             lds = -1;
         }
@@ -502,24 +506,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
 // with each neighbour's T formed once.  img != nullptr fuses the lagged
 // full-mode imaging grad += 2 dt^2 img lambda / v.
 template <int M, int RY, bool BX, bool BY, bool BZ>
-__global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFields f, const float *__restrict__ img,
-                                                   float *__restrict__ grad, ZSet zs)
+__global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFields f, const float *__restrict__ img, float *__restrict__ grad, ZSet zs)
 {
     constexpr bool SHELL = BX || BY || BZ;
     constexpr int TY = 4 * RY;
     using PM = PlaneMap<M, TY>;
-    constexpr int LW = PM::LW, PS = PM::LH * PM::LW;
+    constexpr int LW = PM::LW, PS = PM::LH * PM::LW, YH = PM::YH;
     constexpr int NPL = 1 + (BX ? 3 : 0) + (BY ? 3 : 0);
     constexpr int SX = 1, SY = BX ? 4 : 1;   // first plane slot of (U, V, W) for x, y
+    constexpr int R = 2 * M + 2, H = M + 1;
     __shared__ float lds[2][NPL * PS];
     int b, k, xo, yo, zb, ze;
     if (!zcoords<TY>(zs, b, k, xo, yo, zb, ze)) return;
-    const ZRect R = zs.r[k];
+    const ZRect Rc = zs.r[k];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nx = g.nx, ny = g.ny;
     const int64_t sxy = g.sxy;
-    const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
+    const Lanes<M, RY> L(Rc, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
+    const XHalo<M, RY> xh(lane, w, xo, yo, nx, ny);
     const int64_t off = int64_t(b) * g.n;
     const float dt2 = g.dt2;
     // fields from plane zb - M on; plane z sits at byte offset pl(z)
@@ -534,27 +539,24 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
     const Fld pox(f.psio[0], fb, fe), poy(f.psio[1], fb, fe), poz(f.psio[2], fb, fe);
     const Fld zox(f.zetao[0], fb, fe), zoy(f.zetao[1], fb, fe), zoz(f.zetao[2], fb, fe);
 
-    uint32_t col[RY], xhc[RY], yhc[PM::YH];   // in-plane byte offsets
+    uint32_t col[RY], yhc[YH];   // in-plane byte offsets
 #pragma unroll
-    for (int r = 0; r < RY; ++r) {
-        col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
-        xhc[r] = uint32_t(L.yc[r] * nx + pm.xh_col) * 4u;
-    }
+    for (int r = 0; r < RY; ++r) col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
 #pragma unroll
-    for (int t = 0; t < PM::YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
+    for (int t = 0; t < YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
 
-    // profiles: the lane's column / halo column (x), the row / halo rows (y)
+    // profiles: the lane's column / x-halo duty column (x), rows / halo rows (y)
     float ax = 0.f, bx = 0.f, dbx = 0.f, dax = 0.f, hax = 0.f, hbx = 0.f, hdbx = 0.f;
-    float ay[RY], by[RY], dby[RY], day[RY], hay[PM::YH], hby[PM::YH], hdby[PM::YH];
+    float ay[RY], by[RY], dby[RY], day[RY], hay[YH], hby[YH], hdby[YH];
     if constexpr (BX) {
         const int xs = min(max(L.x, M), nx - 1 - M);
         ax = pr.a[0][L.xc];
         bx = pr.b[0][L.xc];
         dbx = pr.db[0][L.xc];
         dax = d1_taps<M>(c, [&](int q) { return pr.a[0][xs + q]; }, g.inv_dx);
-        hax = pr.a[0][pm.xh_col];
-        hbx = pr.b[0][pm.xh_col];
-        hdbx = pr.db[0][pm.xh_col];
+        hax = pr.a[0][xh.col];
+        hbx = pr.b[0][xh.col];
+        hdbx = pr.db[0][xh.col];
     }
     if constexpr (BY) {
 #pragma unroll
@@ -566,7 +568,7 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
             day[r] = d1_taps<M>(c, [&](int q) { return pr.a[1][ys + q]; }, g.inv_dy);
         }
 #pragma unroll
-        for (int t = 0; t < PM::YH; ++t) {
+        for (int t = 0; t < YH; ++t) {
             const int yr = pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0;
             hay[t] = pr.a[1][yr];
             hby[t] = pr.b[1][yr];
@@ -574,66 +576,6 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
         }
     }
     auto gwof = [&](float lv, float v) { return v * v * dt2 * lv; };
-
-    // queues over planes z-M..z+M: gw (z taps), lambda (centre), U/V/W of the z axis
-    constexpr int Q = 2 * M + 1;
-    float qg[RY][Q], ql[RY][Q];
-    float qu[RY][BZ ? 2 * M + 1 : 1], qv[RY][BZ ? 2 * M + 1 : 1], qw[RY][BZ ? 2 * M + 1 : 1];
-    // plane-(z+1) registers
-    float nl[RY], nvh[RY], nlp[RY];
-    float nzh[RY], nph[RY];                                 // z-queue head raw zeta_z, psi_z
-    float nxl[RY], nxv[RY];                                 // x-halo lambda, vp
-    float nyl[PM::YH], nyv[PM::YH];                         // y-halo lambda, vp
-    float nzx[RY], npx[RY], nzxh[RY], npxh[RY];              // x: centre, x-halo zeta/psi
-    float nzy[RY], npy[RY], nzyh[PM::YH], npyh[PM::YH];      // y: centre, y-halo zeta/psi
-    float nzz[RY], npz[RY];                                 // z: centre zeta/psi
-
-    auto load_plane = [&](int zn) {
-        const uint32_t so = pl(zn);
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            nlp[r] = lp.ld(col[r], so);
-            nxl[r] = ln.ld(xhc[r], so);
-            nxv[r] = vp.ld(xhc[r], so);
-            if constexpr (BX) {
-                nzx[r] = zex.ld(col[r], so);
-                npx[r] = psx.ld(col[r], so);
-                nzxh[r] = zex.ld(xhc[r], so);
-                npxh[r] = psx.ld(xhc[r], so);
-            }
-            if constexpr (BY) {
-                nzy[r] = zey.ld(col[r], so);
-                npy[r] = psy.ld(col[r], so);
-            }
-            if constexpr (BZ) {
-                nzz[r] = zez.ld(col[r], so);
-                npz[r] = psz.ld(col[r], so);
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < PM::YH; ++t) {
-            if (pm.yh_row[t] >= 0) {
-                nyl[t] = ln.ld(yhc[t], so);
-                nyv[t] = vp.ld(yhc[t], so);
-                if constexpr (BY) {
-                    nzyh[t] = zey.ld(yhc[t], so);
-                    npyh[t] = psy.ld(yhc[t], so);
-                }
-            }
-        }
-    };
-    auto load_head = [&](int zh) {   // queue heads at plane zh
-        const uint32_t so = pl(zh);
-#pragma unroll
-        for (int r = 0; r < RY; ++r) {
-            nl[r] = ln.ld(col[r], so);
-            nvh[r] = vp.ld(col[r], so);
-            if constexpr (BZ) {
-                nzh[r] = zez.ld(col[r], so);
-                nph[r] = psz.ld(col[r], so);
-            }
-        }
-    };
     // U, V, W of one cell from its raw values and its profile (b, db, a)
     auto uvw = [&](float zeta, float psi, float gwv, float bb, float dd, float aa, float &U, float &V, float &W) {
         const float T = bb * zeta + (1.0f + bb) * gwv;
@@ -641,7 +583,43 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
         V = bb * psi + dd * T;
         W = aa * T;
     };
-    auto push_head = [&](int zh, int slot) {   // the heads loaded for plane zh enter queue slot `slot`
+
+    // z queues over rounds of R = 2M + 2 planes (slot (S + M + d) mod R holds
+    // plane z + d): gw (z taps), lambda (centre), U/V/W of the z axis; the gw
+    // halo of a plane is formed with its queue head and waits in an
+    // (M + 1)-deep queue (slot p mod (M + 1) holds plane zb + p) until staged.
+    float qg[RY][R], ql[RY][R];
+    float qu[RY][BZ ? R : 1], qv[RY][BZ ? R : 1], qw[RY][BZ ? R : 1];
+    float hgx[H], hgy[H][YH];
+    // heads of plane z + M + 1 in flight
+    float nl[RY], nvh[RY], nzh[RY], nph[RY], nhl = 0.f, nhv = 0.f, nyl[YH], nyv[YH];
+    // single-use fields of plane z + 1 in flight
+    float nlp[RY], nzx[RY], npx[RY], nzxh = 0.f, npxh = 0.f, nzy[RY], npy[RY], nzyh[YH], npyh[YH], nzz[RY], npz[RY];
+
+    auto load_head = [&](int zh) {   // queue heads and the lambda / vp halo of plane zh
+        const uint32_t so = pl(zh);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            nl[r] = ln.ld(col[r], so);
+            nvh[r] = vp.ld(col[r], so);
+            if constexpr (BZ) {
+                nzh[r] = zez.ld_nt(col[r], so);
+                nph[r] = psz.ld_nt(col[r], so);
+            }
+        }
+        if (xh.lds >= 0) {
+            nhl = ln.ld(xh.off, so);
+            nhv = vp.ld(xh.off, so);
+        }
+#pragma unroll
+        for (int t = 0; t < YH; ++t) {
+            if (pm.yh_row[t] >= 0) {
+                nyl[t] = ln.ld(yhc[t], so);
+                nyv[t] = vp.ld(yhc[t], so);
+            }
+        }
+    };
+    auto push_head = [&](int zh, int slot, int hslot) {   // the heads of plane zh into their queue slots
         float bz_ = 0.f, dbz_ = 0.f, az_ = 0.f;
         if constexpr (BZ) {
             bz_ = pr.b[2][zh];
@@ -655,27 +633,57 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
             ql[r][slot] = nl[r];
             if constexpr (BZ) uvw(nzh[r], nph[r], gh, bz_, dbz_, az_, qu[r][slot], qv[r][slot], qw[r][slot]);
         }
+        hgx[hslot] = gwof(nhl, nhv);
+#pragma unroll
+        for (int t = 0; t < YH; ++t) hgy[hslot][t] = gwof(nyl[t], nyv[t]);
     };
-    auto store_plane = [&](float *P, int cs) {   // stage the plane whose centre gw sits in queue slot cs
+    auto load_plane = [&](int zn) {   // single-use fields of plane zn
+        const uint32_t so = pl(zn);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            nlp[r] = lp.ld_nt(col[r], so);
+            if constexpr (BX) {
+                nzx[r] = zex.ld(col[r], so);
+                npx[r] = psx.ld(col[r], so);
+            }
+            if constexpr (BY) {
+                nzy[r] = zey.ld(col[r], so);
+                npy[r] = psy.ld(col[r], so);
+            }
+            if constexpr (BZ) {
+                nzz[r] = zez.ld(col[r], so);
+                npz[r] = psz.ld(col[r], so);
+            }
+        }
+        if constexpr (BX) {
+            if (xh.lds >= 0) {
+                nzxh = zex.ld(xh.off, so);
+                npxh = psx.ld(xh.off, so);
+            }
+        }
+        if constexpr (BY) {
+#pragma unroll
+            for (int t = 0; t < YH; ++t) {
+                if (pm.yh_row[t] >= 0) {
+                    nzyh[t] = zey.ld(yhc[t], so);
+                    npyh[t] = psy.ld(yhc[t], so);
+                }
+            }
+        }
+    };
+    // stage plane z: centre gw from queue slot cs, gw halo from halo slot hs
+    auto store_plane = [&](float *P, int cs, int hs) {
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             const int row = (w + 4 * r + M) * LW;
             const float gc = qg[r][cs];
             P[row + lane + M] = gc;
-            const float gh = gwof(nxl[r], nxv[r]);
-            if (pm.xh_slot >= 0) P[row + pm.xh_slot] = gh;
             if constexpr (BX) {
                 float U, V, W;
                 uvw(nzx[r], npx[r], gc, bx, dbx, ax, U, V, W);
                 P[SX * PS + row + lane + M] = U;
                 P[(SX + 1) * PS + row + lane + M] = V;
                 P[(SX + 2) * PS + row + lane + M] = W;
-                if (pm.xh_slot >= 0) {
-                    uvw(nzxh[r], npxh[r], gh, hbx, hdbx, hax, U, V, W);
-                    P[SX * PS + row + pm.xh_slot] = U;
-                    P[(SX + 1) * PS + row + pm.xh_slot] = V;
-                    P[(SX + 2) * PS + row + pm.xh_slot] = W;
-                }
             }
             if constexpr (BY) {
                 float U, V, W;
@@ -685,11 +693,22 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                 P[(SY + 2) * PS + row + lane + M] = W;
             }
         }
+        if (xh.lds >= 0) {
+            const float gh = hgx[hs];
+            P[xh.lds] = gh;
+            if constexpr (BX) {
+                float U, V, W;
+                uvw(nzxh, npxh, gh, hbx, hdbx, hax, U, V, W);
+                P[SX * PS + xh.lds] = U;
+                P[(SX + 1) * PS + xh.lds] = V;
+                P[(SX + 2) * PS + xh.lds] = W;
+            }
+        }
 #pragma unroll
-        for (int t = 0; t < PM::YH; ++t) {
+        for (int t = 0; t < YH; ++t) {
             if (pm.yh_row[t] >= 0) {
                 const int row = pm.yh_slot[t] * LW + lane + M;
-                const float gh = gwof(nyl[t], nyv[t]);
+                const float gh = hgy[hs][t];
                 P[row] = gh;
                 if constexpr (BY) {
                     float U, V, W;
@@ -702,25 +721,24 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
         }
     };
 
-    // prologue: queue slot t holds plane zb - M + t, plane zb staged
+    // prologue: queue slot t holds plane zb - M + t (t <= 2M), halo slot p
+    // plane zb + p (p <= M); plane zb staged in buffer 0
 #pragma unroll
     for (int t = 0; t <= 2 * M; ++t) {
-        load_head(zb - M + t);
-        push_head(zb - M + t, t);
+        load_head(min(zb - M + t, ze - 1 + M));
+        push_head(min(zb - M + t, ze - 1 + M), t, t >= M ? t - M : 0);
     }
     load_plane(zb);
-    store_plane(lds[zb & 1], M);
+    store_plane(lds[0], M, 0);
     __syncthreads();
 
-    // rounds of Q planes with compile-time queue slots: at step S, plane z + d
-    // sits in slot (S + M + d) mod Q; the slot of plane z - M takes plane z + M + 1
-    for (int z0 = zb; z0 < ze; z0 += Q) {
-        static_for<0, Q>([&](auto Sc) -> bool {
+    for (int z0 = zb; z0 < ze; z0 += R) {
+        static_for<0, R>([&](auto Sc) -> bool {
         constexpr int S = decltype(Sc)::value;
-        constexpr int C = (S + M) % Q;
+        constexpr int C = (S + M) % R;
         const int z = z0 + S;
         if (z >= ze) return false;
-        const float *P = lds[z & 1];
+        const float *P = lds[S & 1];
         float clp[RY], czx[RY], cpx[RY], czy[RY], cpy[RY], czz[RY], cpz[RY];
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
@@ -732,11 +750,9 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
             czz[r] = nzz[r];
             cpz[r] = npz[r];
         }
-        {
-            const int zn = min(z + 1, ze - 1);
-            load_head(zn + M);
-            load_plane(zn);
-        }
+        const int zh = min(z + M + 1, ze - 1 + M);   // clamped at the segment end, unused there
+        load_head(zh);
+        load_plane(min(z + 1, ze - 1));
         const uint32_t so = pl(z);
         float az = 0.f, bz = 0.f, daz = 0.f;
         if constexpr (BZ) {
@@ -751,6 +767,7 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
             const float *Gw = P + row;
             const float gun = ql[r][C];
             const float gw = qg[r][C];
+            auto zq = [&](const float (&qq)[R], int t) { return qq[(C + t + R) % R]; };
             if (grad && img && L.x >= g.X0 && L.x < g.X1 && L.y[r] >= g.Y0 && L.y[r] < g.Y1 && z >= g.Z0 &&
                 z < g.Z1) {
                 const int64_t i = off + int64_t(z) * sxy + col[r] / 4u;
@@ -763,12 +780,11 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                 for (int t = 1; t <= M; ++t) {
                     lx += c.lap[t] * ((Gw[t] - gw) + (Gw[-t] - gw));
                     ly += c.lap[t] * ((Gw[t * LW] - gw) + (Gw[-t * LW] - gw));
-                    lz += c.lap[t] * ((qg[r][(C + t) % Q] - gw) + (qg[r][(C - t + Q) % Q] - gw));
+                    lz += c.lap[t] * ((zq(qg[r], t) - gw) + (zq(qg[r], -t) - gw));
                 }
                 out = 2.0f * gun - clp[r] + lx * g.inv_dx2 + ly * g.inv_dy2 + lz * g.inv_dz2;
             } else {
                 float Lt[3], Dg[3];
-                // x
                 if constexpr (BX) {
                     const float *Ux = P + SX * PS + row, *Vx = P + (SX + 1) * PS + row, *Wx = P + (SX + 2) * PS + row;
                     const float gt0 = bx * czx[r] + (1.0f + bx) * gw;
@@ -783,8 +799,8 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                     Lt[0] = Ls * g.inv_dx2;
                     Dg[0] = Dgg * g.inv_dx;
                     Dq *= g.inv_dx;
-                    pox.st(ax * cpx[r] + dax * gt0 - Dq, col[r], so);
-                    zox.st(ax * (czx[r] + gw), col[r], so);
+                    pox.st_nt(ax * cpx[r] + dax * gt0 - Dq, col[r], so);
+                    zox.st_nt(ax * (czx[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
@@ -792,7 +808,6 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                     Lt[0] = Ls * g.inv_dx2;
                     Dg[0] = 0.f;
                 }
-                // y
                 if constexpr (BY) {
                     const float *Uy = P + SY * PS + row, *Vy = P + (SY + 1) * PS + row, *Wy = P + (SY + 2) * PS + row;
                     const float gt0 = by[r] * czy[r] + (1.0f + by[r]) * gw;
@@ -807,8 +822,8 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                     Lt[1] = Ls * g.inv_dy2;
                     Dg[1] = Dgg * g.inv_dy;
                     Dq *= g.inv_dy;
-                    poy.st(ay[r] * cpy[r] + day[r] * gt0 - Dq, col[r], so);
-                    zoy.st(ay[r] * (czy[r] + gw), col[r], so);
+                    poy.st_nt(ay[r] * cpy[r] + day[r] * gt0 - Dq, col[r], so);
+                    zoy.st_nt(ay[r] * (czy[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
@@ -816,36 +831,34 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
                     Lt[1] = Ls * g.inv_dy2;
                     Dg[1] = 0.f;
                 }
-                // z
                 if constexpr (BZ) {
                     const float gt0 = bz * czz[r] + (1.0f + bz) * gw;
                     const float l0 = (1.0f + bz) * gt0;
                     float Ls = 0.f, Dgg = 0.f, Dq = 0.f;
 #pragma unroll
                     for (int t = 1; t <= M; ++t) {
-                        Ls += c.lap[t] * ((qu[r][(C + t) % Q] - l0) + (qu[r][(C - t + Q) % Q] - l0));
-                        Dgg += c.grad[t] * (qv[r][(C + t) % Q] - qv[r][(C - t + Q) % Q]);
-                        Dq += c.grad[t] * (qw[r][(C + t) % Q] - qw[r][(C - t + Q) % Q]);
+                        Ls += c.lap[t] * ((zq(qu[r], t) - l0) + (zq(qu[r], -t) - l0));
+                        Dgg += c.grad[t] * (zq(qv[r], t) - zq(qv[r], -t));
+                        Dq += c.grad[t] * (zq(qw[r], t) - zq(qw[r], -t));
                     }
                     Lt[2] = Ls * g.inv_dz2;
                     Dg[2] = Dgg * g.inv_dz;
                     Dq *= g.inv_dz;
-                    poz.st(az * cpz[r] + daz * gt0 - Dq, col[r], so);
-                    zoz.st(az * (czz[r] + gw), col[r], so);
+                    poz.st_nt(az * cpz[r] + daz * gt0 - Dq, col[r], so);
+                    zoz.st_nt(az * (czz[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
-                    for (int t = 1; t <= M; ++t)
-                        Ls += c.lap[t] * ((qg[r][(C + t) % Q] - gw) + (qg[r][(C - t + Q) % Q] - gw));
+                    for (int t = 1; t <= M; ++t) Ls += c.lap[t] * ((zq(qg[r], t) - gw) + (zq(qg[r], -t) - gw));
                     Lt[2] = Ls * g.inv_dz2;
                     Dg[2] = 0.f;
                 }
                 out = 2.0f * gun - clp[r] + (Lt[0] + Lt[1] + Lt[2] - Dg[0] - Dg[1] - Dg[2]);
             }
-            lnx.st(out, col[r], so);
+            lnx.st_nt(out, col[r], so);
         }
-        push_head(min(z + 1, ze - 1) + M, S);
-        if (z + 1 < ze) store_plane(lds[(z + 1) & 1], (C + 1) % Q);
+        push_head(zh, (S + 2 * M + 1) % R, S % H);
+        if (z + 1 < ze) store_plane(lds[(S + 1) & 1], (C + 1) % R, (S + 1) % H);
         __syncthreads();
         return true;
         });
