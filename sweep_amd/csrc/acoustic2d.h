@@ -16,8 +16,10 @@ struct Geo {
     int abcn;
     int W;                     // boundary strip width M + 1
     int nxp, nzp;              // physical box extents
-    int64_t n;                 // cells per shot
-    int64_t mstride;           // model stride per shot (0: shared model)
+    int px;                    // row pitch of the core's arrays (nx rounded up to 32)
+    int64_t n;                 // cells per shot in the core's arrays (nz * px)
+    int64_t nm;                // cells per shot in the caller's arrays (nz * nx)
+    int64_t mstride;           // model stride per shot in the core's copy (0: shared model)
     int64_t ns;                // strip cells per shot per step
     float dt, dx, dz;
     float inv_dx2, inv_dz2, inv_dx, inv_dz, dt2, inv_dt2;
@@ -62,6 +64,7 @@ struct Rect {
 // Up to four rectangles enumerated back to back by blockIdx.x.
 struct BlockSet {
     Rect r[4];
+    int x0;   // first column of strip 0 (0: strips start on 128-byte lines)
 };
 
 // Tiling of the computed box [M, n-M) into 64-column strips x row

@@ -37,7 +37,10 @@ Geo make_geo(const swp_problem &p)
     g.dt = p.dt;
     g.dx = p.dx;
     g.dz = p.dz;
-    g.n = int64_t(g.nz) * g.nx;
+    // rows of every array the core allocates start on 128-byte lines
+    g.px = (g.nx + 31) / 32 * 32;
+    g.n = int64_t(g.nz) * g.px;
+    g.nm = int64_t(g.nz) * g.nx;
     g.mstride = p.model_batched ? g.n : 0;
     g.W = g.M + 1;
     g.nxp = g.X1 - g.X0;
@@ -168,10 +171,10 @@ __global__ void __launch_bounds__(256) fwd_kernel(Geo g, Coef c, Prof pr, StepFi
     const int b = blockIdx.z;
     if (ix >= g.nx - M || iz >= g.nz - M) return;
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = int64_t(iz) * g.nx + ix;
+    const int64_t i = int64_t(iz) * g.px + ix;
     const float *__restrict__ un = f.u_now + off;
     const float *__restrict__ vp = f.vp + int64_t(b) * g.mstride;
-    const int64_t nx = g.nx;
+    const int64_t nx = g.px;   // row pitch
 
     if (strip) {   // boundary save of u_now (driver_traits.cuh:248-265, strip.cuh)
         float *s = strip + int64_t(b) * g.ns;
@@ -246,8 +249,8 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
     const int b = blockIdx.z;
     if (ix >= g.nx - M || iz >= g.nz - M) return;
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = int64_t(iz) * g.nx + ix;
-    const int64_t nx = g.nx;
+    const int64_t i = int64_t(iz) * g.px + ix;
+    const int64_t nx = g.px;   // row pitch
     const float *__restrict__ ln = f.l_now + off;
     const float *__restrict__ vp = f.vp + int64_t(b) * g.mstride;
     const float dt2 = g.dt2;
@@ -338,10 +341,10 @@ __global__ void __launch_bounds__(256) recon_kernel(Geo g, Coef c, const float *
     const int b = blockIdx.z;
     if (ix >= x_hi || iz >= z_hi) return;
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = int64_t(iz) * g.nx + ix;
+    const int64_t i = int64_t(iz) * g.px + ix;
     const float *__restrict__ un = u_now + off;
     const float *__restrict__ vp = vpb + int64_t(b) * g.mstride;
-    const float w_sum = lap_fwd<MS>(c, un, i, 1, g.inv_dx2, M) + lap_fwd<MS>(c, un, i, g.nx, g.inv_dz2, M);
+    const float w_sum = lap_fwd<MS>(c, un, i, 1, g.inv_dx2, M) + lap_fwd<MS>(c, un, i, g.px, g.inv_dz2, M);
     const float v = vp[i];
     const float u0 = un[i];
     const float up = u_prev[off + i];
@@ -389,7 +392,7 @@ __global__ void __launch_bounds__(256) strip_restore_kernel(
         if (l[0] == ix && l[1] == iz) val += -src[(int64_t(b) * nsrc + k) * g.nt + it];
     }
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = int64_t(iz) * g.nx + ix;
+    const int64_t i = int64_t(iz) * g.px + ix;
     u_next[off + i] = val;
     if (grad) {
         const float u_tt = (val - 2 * u_now[off + i] + u_prev[off + i]) * g.inv_dt2;
@@ -413,7 +416,7 @@ __global__ void __launch_bounds__(256) image_kernel(Geo g, const float *__restri
     const int b = blockIdx.z;
     if (ix >= g.X1 || iz >= g.Z1) return;
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = int64_t(iz) * g.nx + ix;
+    const int64_t i = int64_t(iz) * g.px + ix;
     const float l = lam[off + i];
     if (grad) grad[off + i] += 2.f * g.dt * g.dt * img[off + i] * l / vpb[int64_t(b) * g.mstride + i];
     if (R) R[off + i] += l * l;
@@ -435,7 +438,7 @@ __global__ void __launch_bounds__(256) zero_rim_kernel(Geo g, float *__restrict_
     if (ix >= g.nx || iz >= g.nz) return;
     const int w = g.abcn + g.M;
     if (ix < w || ix >= g.nx - w || iz < w || iz >= g.nz - w)
-        u[int64_t(b) * g.n + int64_t(iz) * g.nx + ix] = 0.f;
+        u[int64_t(b) * g.n + int64_t(iz) * g.px + ix] = 0.f;
 }
 
 __global__ void __launch_bounds__(256) copy_kernel(float *__restrict__ dst,
@@ -578,6 +581,35 @@ static Tape carve_tape(Carver &cv, const swp_problem &p, const Geo &g)
     return t;
 }
 
+// The core's row-pitched copies of the caller's model and of the gradient
+// and illumination outputs (the caller's arrays are (B|1, nz, nx) dense).
+struct IoBufs {
+    float *vp = nullptr;                           // (B or 1, n)
+    float *grad = nullptr, *S = nullptr, *R = nullptr;   // (B, n), backward only
+};
+
+static IoBufs carve_io(Carver &cv, const swp_problem &p, const Geo &g, int kind)
+{
+    IoBufs b;
+    b.vp = cv.take<float>((p.model_batched ? int64_t(g.B) : 1) * g.n);
+    if (kind == 1) {
+        const int64_t N = int64_t(g.B) * g.n;
+        b.grad = cv.take<float>(N);
+        b.S = cv.take<float>(N);
+        b.R = cv.take<float>(N);
+    }
+    return b;
+}
+
+// q = p with the model pointer on the core's pitched copy (filled here).
+static swp_problem with_pitched_model(const swp_problem &p, const Geo &g, const IoBufs &b)
+{
+    swp_problem q = p;
+    pitch_copy(b.vp, g.px, p.models[0], g.nx, (p.model_batched ? int64_t(g.B) : 1) * g.nz, g.nx);
+    q.models[0] = b.vp;
+    return q;
+}
+
 static PointArgs point_args(const Geo &g)
 {
     PointArgs a{};
@@ -586,6 +618,7 @@ static PointArgs point_args(const Geo &g)
     a.nz = g.nz;
     a.ny = 1;
     a.nx = g.nx;
+    a.px = g.px;
     a.n = g.n;
     a.sign = 1.0f;
     return a;
@@ -645,8 +678,9 @@ int64_t workspace_bytes(const swp_problem &p, int kind)
     if (kind == 1) {
         AdjState as;
         carve_adj(cv, g, as);
-        if (p.memory == SWP_MEM_CKPT) cv.take<float>(int64_t(p.ckpt_interval) * g.B * g.n);
     }
+    carve_io(cv, p, g, kind);
+    if (kind == 1 && p.memory == SWP_MEM_CKPT) cv.take<float>(int64_t(p.ckpt_interval) * g.B * g.n);
     return cv.used + 256;
 }
 
@@ -658,13 +692,14 @@ int64_t tape_bytes(const swp_problem &p)
     return cv.used + 256;
 }
 
-void forward(const Ctx &x, const swp_problem &p, float *record)
+void forward(const Ctx &x, const swp_problem &p_in, float *record)
 {
     const Geo &g = x.g;
-    Carver ws(p.workspace, p.workspace_bytes);
+    Carver ws(p_in.workspace, p_in.workspace_bytes);
     FwdState s;
     carve_fwd(ws, g, s);
     zero_fwd(g, s);
+    const swp_problem p = with_pitched_model(p_in, g, carve_io(ws, p_in, g, 0));
     Carver tc(p.tape, p.tape_bytes);
     Tape t = carve_tape(tc, p, g);
     const int64_t N = int64_t(g.B) * g.n;
@@ -726,16 +761,39 @@ static void image(const Ctx &x, const swp_problem &p, const float *img, const fl
     SWP_LAUNCHED();
 }
 
-void backward(const Ctx &x, const swp_problem &p, const swp_backward_io &io)
+static void backward_run(const Ctx &x, const swp_problem &p, const swp_backward_io &io, Carver &ws,
+                         FwdState &fs, AdjState &a);
+
+void backward(const Ctx &x, const swp_problem &p_in, const swp_backward_io &io_in)
 {
     const Geo &g = x.g;
-    const int64_t N = int64_t(g.B) * g.n;
-    Carver ws(p.workspace, p.workspace_bytes);
+    Carver ws(p_in.workspace, p_in.workspace_bytes);
     FwdState fs;
     carve_fwd(ws, g, fs);
     AdjState a;
     carve_adj(ws, g, a);
     zero_adj(g, a);
+    const IoBufs b = carve_io(ws, p_in, g, 1);
+    const swp_problem p = with_pitched_model(p_in, g, b);
+    // gradients and illumination accumulate in the core's pitched buffers,
+    // then add into the caller's
+    swp_backward_io io = io_in;
+    const int64_t N = int64_t(g.B) * g.n, rows = int64_t(g.B) * g.nz;
+    const bool illum = (p.flags & 1) != 0;
+    if (io_in.grad_models[0]) { fill_zero(b.grad, N); io.grad_models[0] = b.grad; }
+    if (illum && io_in.illum_source) { fill_zero(b.S, N); io.illum_source = b.S; }
+    if (illum && io_in.illum_receiver) { fill_zero(b.R, N); io.illum_receiver = b.R; }
+    backward_run(x, p, io, ws, fs, a);
+    if (io_in.grad_models[0]) pitch_add(io_in.grad_models[0], g.nx, b.grad, g.px, rows, g.nx);
+    if (illum && io_in.illum_source) pitch_add(io_in.illum_source, g.nx, b.S, g.px, rows, g.nx);
+    if (illum && io_in.illum_receiver) pitch_add(io_in.illum_receiver, g.nx, b.R, g.px, rows, g.nx);
+}
+
+static void backward_run(const Ctx &x, const swp_problem &p, const swp_backward_io &io, Carver &ws,
+                         FwdState &fs, AdjState &a)
+{
+    const Geo &g = x.g;
+    const int64_t N = int64_t(g.B) * g.n;
     Carver tc(p.tape, p.tape_bytes);
     Tape t = carve_tape(tc, p, g);
     float *grad = io.grad_models[0];

@@ -81,11 +81,11 @@ __global__ void __launch_bounds__(64) fwd_int_kernel(Geo g, Coef c, StepFields f
     block_coords(bs, strip, zb, ze);
     if (ze <= zb) return;
     const int lane = threadIdx.x;
-    const int xb = M + strip * 64;
+    const int xb = bs.x0 + strip * 64;
     const int x = xb + lane;
-    const int nx = g.nx;
+    const int nx = g.nx, px = g.px;
     const int xc = min(x, nx - 1);
-    const bool active = x < nx - M;
+    const bool active = x >= M && x < nx - M;
     const int64_t off = int64_t(blockIdx.z) * g.n;
     const float *__restrict__ un = f.u_now + off;
     const float *__restrict__ up = f.u_prev + off;
@@ -96,20 +96,20 @@ __global__ void __launch_bounds__(64) fwd_int_kernel(Geo g, Coef c, StepFields f
 
     float q[2 * M + 1];
 #pragma unroll
-    for (int k = 0; k < 2 * M; ++k) q[k] = un[int64_t(zb - M + k) * nx + xc];
+    for (int k = 0; k < 2 * M; ++k) q[k] = un[int64_t(zb - M + k) * px + xc];
     // row zb in flight
-    float n_q = un[int64_t(zb + M) * nx + xc];
-    float n_h = un[int64_t(zb) * nx + h.col];
-    float n_up = up[int64_t(zb) * nx + xc];
-    float n_v = vp[int64_t(zb) * nx + xc];
+    float n_q = un[int64_t(zb + M) * px + xc];
+    float n_h = un[int64_t(zb) * px + h.col];
+    float n_up = up[int64_t(zb) * px + xc];
+    float n_v = vp[int64_t(zb) * px + xc];
     for (int z = zb; z < ze; ++z) {
-        const int64_t rz = int64_t(z) * nx;
+        const int64_t rz = int64_t(z) * px;
         q[2 * M] = n_q;
         const float hv = n_h, upv = n_up, v = n_v;
         {   // issue row z+1 (clamped on the last row)
             const int zn = min(z + 1, ze - 1);
-            const int64_t rn = int64_t(zn) * nx;
-            n_q = un[rn + int64_t(M) * nx + xc];
+            const int64_t rn = int64_t(zn) * px;
+            n_q = un[rn + int64_t(M) * px + xc];
             n_h = un[rn + h.col];
             n_up = up[rn + xc];
             n_v = vp[rn + xc];
@@ -141,11 +141,11 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
     block_coords(bs, strip, zb, ze);
     if (ze <= zb) return;
     const int lane = threadIdx.x;
-    const int xb = M + strip * 64;
+    const int xb = bs.x0 + strip * 64;
     const int x = xb + lane;
-    const int nx = g.nx, nz = g.nz;
+    const int nx = g.nx, px = g.px, nz = g.nz;
     const int xc = min(x, nx - 1);
-    const bool active = x < nx - M;
+    const bool active = x >= M && x < nx - M;
     const int b = blockIdx.z;
     const int64_t off = int64_t(b) * g.n;
     const float *__restrict__ un = f.u_now + off;
@@ -176,17 +176,17 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
 #pragma unroll
     for (int k = 0; k < 2 * M; ++k) {
         const int zr = zb - M + k;
-        q[k] = un[int64_t(zr) * nx + xc];
-        pq[k] = seg_z ? pr.az[zr] * psiz[int64_t(zr) * nx + xc] : 0.f;
+        q[k] = un[int64_t(zr) * px + xc];
+        pq[k] = seg_z ? pr.az[zr] * psiz[int64_t(zr) * px + xc] : 0.f;
     }
     float *s_out = STRIP ? strip_out + int64_t(b) * g.ns : nullptr;
 
     float n_q, n_pq = 0.f, n_h, n_hp = 0.f, n_px = 0.f, n_up, n_v, n_zx = 0.f, n_zz = 0.f, n_pz = 0.f;
     auto issue = [&](int zr) {
-        const int64_t rn = int64_t(zr) * nx;
-        n_q = un[rn + int64_t(M) * nx + xc];
+        const int64_t rn = int64_t(zr) * px;
+        n_q = un[rn + int64_t(M) * px + xc];
         if (seg_z) {
-            n_pq = pr.az[zr + M] * psiz[rn + int64_t(M) * nx + xc];
+            n_pq = pr.az[zr + M] * psiz[rn + int64_t(M) * px + xc];
             n_zz = zetaz[rn + xc];
             n_pz = psiz[rn + xc];
         }
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
     };
     issue(zb);
     for (int z = zb; z < ze; ++z) {
-        const int64_t rz = int64_t(z) * nx;
+        const int64_t rz = int64_t(z) * px;
         q[2 * M] = n_q;
         pq[2 * M] = n_pq;
         const float hv = n_h, hp = n_hp, pxv = n_px, upv = n_up, v = n_v, zxc = n_zx, zzc = n_zz, pzc = n_pz;
@@ -281,11 +281,11 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
     block_coords(bs, strip, zb, ze);
     if (ze <= zb) return;
     const int lane = threadIdx.x;
-    const int xb = M + strip * 64;
+    const int xb = bs.x0 + strip * 64;
     const int x = xb + lane;
-    const int nx = g.nx;
+    const int nx = g.nx, px = g.px;
     const int xc = min(x, nx - 1);
-    const bool active = x < nx - M;
+    const bool active = x >= M && x < nx - M;
     const int b = blockIdx.z;
     const int64_t off = int64_t(b) * g.n;
     const float *__restrict__ ln = f.l_now + off;
@@ -298,15 +298,15 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
     float q[2 * M + 1];   // gw = (v^2 dt^2) lambda
 #pragma unroll
     for (int k = 0; k < 2 * M; ++k) {
-        const int64_t r = int64_t(zb - M + k) * nx + xc;
+        const int64_t r = int64_t(zb - M + k) * px + xc;
         const float v = vp[r];
         q[k] = v * v * dt2 * ln[r];
     }
     float n_v, n_l, n_hv, n_hl, n_lp, n_im = 0.f;
     auto issue = [&](int zr) {
-        const int64_t rn = int64_t(zr) * nx;
-        n_v = vp[rn + int64_t(M) * nx + xc];
-        n_l = ln[rn + int64_t(M) * nx + xc];
+        const int64_t rn = int64_t(zr) * px;
+        n_v = vp[rn + int64_t(M) * px + xc];
+        n_l = ln[rn + int64_t(M) * px + xc];
         n_hv = vp[rn + h.col];
         n_hl = ln[rn + h.col];
         n_lp = lp[rn + xc];
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
     };
     issue(zb);
     for (int z = zb; z < ze; ++z) {
-        const int64_t rz = int64_t(z) * nx;
+        const int64_t rz = int64_t(z) * px;
         q[2 * M] = n_v * n_v * dt2 * n_l;
         const float hv = n_hv * n_hv * dt2 * n_hl;
         const float lpv = n_lp, im = n_im;
@@ -352,11 +352,11 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
     block_coords(bs, strip, zb, ze);
     if (ze <= zb) return;
     const int lane = threadIdx.x;
-    const int xb = M + strip * 64;
+    const int xb = bs.x0 + strip * 64;
     const int x = xb + lane;
-    const int nx = g.nx;
+    const int nx = g.nx, px = g.px;
     const int xc = min(x, nx - 1);
-    const bool active = x < nx - M;
+    const bool active = x >= M && x < nx - M;
     const int b = blockIdx.z;
     const int64_t off = int64_t(b) * g.n;
     const float *__restrict__ ln = f.l_now + off;
@@ -397,18 +397,18 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
 #pragma unroll
     for (int k = 0; k < 2 * M; ++k) {
         const int zr = zb - M + k;
-        const int64_t r = int64_t(zr) * nx + xc;
+        const int64_t r = int64_t(zr) * px + xc;
         put_row(zr, k, vp[r], ln[r], seg_z ? zetaz[r] : 0.f, seg_z ? psiz[r] : 0.f);
     }
     // row z+M (queue tail) and row z (halo, own x quantities) in flight
     float n_v, n_l, n_zz = 0.f, n_pz = 0.f, n_hv, n_hl, n_hz = 0.f, n_hp = 0.f;
     float n_gun, n_lp, n_v0, n_zx = 0.f, n_px = 0.f, n_im = 0.f;
     auto issue = [&](int zr) {
-        const int64_t rq = int64_t(zr + M) * nx + xc;
+        const int64_t rq = int64_t(zr + M) * px + xc;
         n_v = vp[rq];
         n_l = ln[rq];
         if (seg_z) { n_zz = zetaz[rq]; n_pz = psiz[rq]; }
-        const int64_t rn = int64_t(zr) * nx;
+        const int64_t rn = int64_t(zr) * px;
         n_hv = vp[rn + h.col];
         n_hl = ln[rn + h.col];
         if (wave_x) { n_hz = zetax[rn + h.col]; n_hp = psix[rn + h.col]; n_zx = zetax[rn + xc]; n_px = psix[rn + xc]; }
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
     };
     issue(zb);
     for (int z = zb; z < ze; ++z) {
-        const int64_t rz = int64_t(z) * nx;
+        const int64_t rz = int64_t(z) * px;
         put_row(z + M, 2 * M, n_v, n_l, n_zz, n_pz);
         const float hvv = n_hv, hl = n_hl, hz = n_hz, hpx = n_hp;
         const float gun = n_gun, lpv = n_lp, v0 = n_v0, zx = n_zx, px = n_px, im = n_im;
@@ -518,7 +518,8 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
 
 // ---------------------------------------------------------------- BS recon
 // Reverse leapfrog without PML over [x_lo, x_hi) x [z_lo, z_hi), fused u_tt
-// imaging inside the strips' inner edge.  Strips of 64 columns start at x_lo.
+// imaging inside the strips' inner edge.  Strips of 64 columns start at x_lo rounded
+// down to a 128-byte line; lanes outside [x_lo, x_hi) only stage their column.
 template <int M, bool IMG>
 __global__ void __launch_bounds__(64) recon_zm_kernel(Geo g, Coef c, const float *__restrict__ vpb,
                                                       const float *__restrict__ u_prev,
@@ -530,11 +531,11 @@ __global__ void __launch_bounds__(64) recon_zm_kernel(Geo g, Coef c, const float
 {
     __shared__ float row[64 + 2 * M];
     const int lane = threadIdx.x;
-    const int xb = x_lo + int(blockIdx.x) * 64;
+    const int xb = x_lo - x_lo % 32 + int(blockIdx.x) * 64;   // strips on 128-byte lines
     const int x = xb + lane;
-    const int nx = g.nx;
+    const int nx = g.nx, px = g.px;
     const int xc = min(x, nx - 1);
-    const bool active = x < x_hi;
+    const bool active = x >= x_lo && x < x_hi;
     const int zb = z_lo + int(blockIdx.y) * tz;
     const int ze = min(zb + tz, z_hi);
     const int b = blockIdx.z;
@@ -548,11 +549,11 @@ __global__ void __launch_bounds__(64) recon_zm_kernel(Geo g, Coef c, const float
 
     float q[2 * M + 1];
 #pragma unroll
-    for (int k = 0; k < 2 * M; ++k) q[k] = un[int64_t(zb - M + k) * nx + xc];
+    for (int k = 0; k < 2 * M; ++k) q[k] = un[int64_t(zb - M + k) * px + xc];
     float n_q, n_h, n_up, n_v, n_a = 0.f;
     auto issue = [&](int zr) {
-        const int64_t rn = int64_t(zr) * nx;
-        n_q = un[rn + int64_t(M) * nx + xc];
+        const int64_t rn = int64_t(zr) * px;
+        n_q = un[rn + int64_t(M) * px + xc];
         n_h = un[rn + h.col];
         n_up = up[rn + xc];
         n_v = vp[rn + xc];
@@ -560,7 +561,7 @@ __global__ void __launch_bounds__(64) recon_zm_kernel(Geo g, Coef c, const float
     };
     issue(zb);
     for (int z = zb; z < ze; ++z) {
-        const int64_t rz = int64_t(z) * nx;
+        const int64_t rz = int64_t(z) * px;
         q[2 * M] = n_q;
         const float hv = n_h, upv = n_up, v = n_v, a = n_a;
         issue(min(z + 1, ze - 1));
@@ -598,15 +599,17 @@ ZmPlan make_plan(const Geo &g, int tz, int tz_band)
     const int M = g.M;
     p.tz = tz;
     p.tz_band = tz_band;
-    p.nstrip = (g.nx - 2 * M + 63) / 64;
+    const int x0 = 0;   // strips start on 128-byte lines (rows are line-aligned)
+    p.nstrip = (g.nx - M - x0 + 63) / 64;
+    p.inner.x0 = p.band.x0 = x0;
     const int zmin = M, zmax = g.nz - M;
     // interior: strips and rows inside [lo, hi) with lo = pad + 2M + 1 on
     // every face (no CPML term, no strip save, no db leak cell reaches them)
     const int xlo = std::max(g.pxl + 2 * M + 1, g.X0 + g.W), xhi = std::min(g.nx - g.pxh - 2 * M - 1, g.X1 - g.W);
     const int zlo = std::max(g.pzl + 2 * M + 1, g.Z0 + g.W), zhi = std::min(g.nz - g.pzh - 2 * M - 1, g.Z1 - g.W);
     int s0 = 0, s1 = p.nstrip;
-    while (s0 < p.nstrip && M + s0 * 64 < xlo) ++s0;
-    while (s1 > s0 && M + s1 * 64 > xhi) --s1;
+    while (s0 < p.nstrip && x0 + s0 * 64 < xlo) ++s0;
+    while (s1 > s0 && x0 + s1 * 64 > xhi) --s1;
     const int zi0 = std::max(zlo, zmin);
     const int zi1 = zi0 + std::max(0, (zhi - zi0) / tz) * tz;   // whole interior segments
     if (s1 <= s0 || zi1 <= zi0) {
@@ -688,7 +691,8 @@ void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const f
                      const float *adj, float *grad, int x_lo, int x_hi, int z_lo, int z_hi, int tz)
 {
     const Geo &g = x.g;
-    const dim3 grid(unsigned((x_hi - x_lo + 63) / 64), unsigned((z_hi - z_lo + tz - 1) / tz), unsigned(g.B));
+    const dim3 grid(unsigned((x_hi - (x_lo - x_lo % 32) + 63) / 64), unsigned((z_hi - z_lo + tz - 1) / tz),
+                    unsigned(g.B));
     by_m(g.M, [&](auto mc) {
         constexpr int M = decltype(mc)::value;
         if (grad) hipLaunchKernelGGL((recon_zm_kernel<M, true>), grid, dim3(64), 0, stream(), g, x.c, vp, u_prev, u_now, u_next, adj, grad, x_lo, x_hi, z_lo, z_hi, tz);

@@ -21,10 +21,18 @@ struct PointArgs {
     int get_accumulate;
     int it, nt, ndim;
     int nz, ny, nx;
+    int px;      // row pitch of u_add / u_get (0: nx)
     int64_t n;   // cells per shot
 };
 
 void launch_points(const PointArgs &a, int B);
+
+// Row-pitched copies between the caller's arrays and the core's: rows rows
+// of width floats, dst row r at dst + r * dpitch, src row r at src + r * spitch.
+// pitch_copy writes dst (and zeroes dst's columns [width, dpitch)),
+// pitch_add adds src into dst.
+void pitch_copy(float *dst, int64_t dpitch, const float *src, int64_t spitch, int64_t rows, int width);
+void pitch_add(float *dst, int64_t dpitch, const float *src, int64_t spitch, int64_t rows, int width);
 
 // Whether equation 0 (acoustic2d), 1 (acoustic3d), 2 (elastic2d) is built.
 bool eq_available(int eq);

@@ -2,6 +2,8 @@
 // inject sources and gather receivers, and the kernel-class timer.
 #include "common.h"
 
+#include <algorithm>
+
 namespace swp {
 
 // ----------------------------------------------------------------- fill
@@ -45,7 +47,7 @@ __global__ void __launch_bounds__(256) points_kernel(PointArgs a)
             const int32_t *l = a.add_loc + (int64_t(b) * a.nadd + s) * a.ndim;
             int ix = l[0], iy = a.ndim == 3 ? l[1] : 0, iz = l[a.ndim - 1];
             if (ix < 0 || ix >= a.nx || iy < 0 || iy >= a.ny || iz < 0 || iz >= a.nz) continue;
-            const int64_t cell = (int64_t(iz) * a.ny + iy) * a.nx + ix;
+            const int64_t cell = (int64_t(iz) * a.ny + iy) * (a.px > 0 ? a.px : a.nx) + ix;
             atomicAdd(u_add + cell, a.sign * a.add_w[(int64_t(b) * a.nadd + s) * a.nt + a.it]);
         }
     }
@@ -56,7 +58,7 @@ __global__ void __launch_bounds__(256) points_kernel(PointArgs a)
         const int32_t *l = a.get_loc + (int64_t(b) * a.nget + r) * a.ndim;
         int ix = l[0], iy = a.ndim == 3 ? l[1] : 0, iz = l[a.ndim - 1];
         if (ix < 0 || ix >= a.nx || iy < 0 || iy >= a.ny || iz < 0 || iz >= a.nz) continue;
-        const int64_t cell = (int64_t(iz) * a.ny + iy) * a.nx + ix;
+        const int64_t cell = (int64_t(iz) * a.ny + iy) * (a.px > 0 ? a.px : a.nx) + ix;
         const float v = __hip_atomic_load(u_get + cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         float *o = a.get_out + (int64_t(b) * a.nget + r) * a.nt + a.it;
         if (a.get_accumulate) *o += v;
@@ -70,6 +72,42 @@ void launch_points(const PointArgs &a, int B)
     if (!a.u_add && !a.u_get) return;
     hipLaunchKernelGGL(points_kernel, dim3(B), dim3(256), 0, stream(), a);
     SWP_LAUNCHED();
+}
+
+// ----------------------------------------------------------------- pitch
+template <bool ADD>
+__global__ void __launch_bounds__(256) pitch_kernel(float *__restrict__ dst, int64_t dp,
+                                                    const float *__restrict__ src, int64_t sp, int64_t rows,
+                                                    int width)
+{
+    const int64_t cols = ADD ? width : dp;
+    const int64_t total = rows * cols;
+    for (int64_t k = int64_t(blockIdx.x) * 256 + threadIdx.x; k < total; k += int64_t(gridDim.x) * 256) {
+        const int64_t r = k / cols, c = k % cols;
+        if (ADD) dst[r * dp + c] += src[r * sp + c];
+        else dst[r * dp + c] = c < width ? src[r * sp + c] : 0.f;
+    }
+}
+
+template <bool ADD>
+static void launch_pitch(float *dst, int64_t dp, const float *src, int64_t sp, int64_t rows, int width)
+{
+    const int64_t total = rows * (ADD ? width : dp);
+    if (total <= 0) return;
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 8192);
+    hipLaunchKernelGGL(pitch_kernel<ADD>, dim3(unsigned(blocks)), dim3(256), 0, stream(), dst, dp, src, sp, rows,
+                       width);
+    SWP_LAUNCHED();
+}
+
+void pitch_copy(float *dst, int64_t dpitch, const float *src, int64_t spitch, int64_t rows, int width)
+{
+    launch_pitch<false>(dst, dpitch, src, spitch, rows, width);
+}
+
+void pitch_add(float *dst, int64_t dpitch, const float *src, int64_t spitch, int64_t rows, int width)
+{
+    launch_pitch<true>(dst, dpitch, src, spitch, rows, width);
 }
 
 // ----------------------------------------------------------------- timer
