@@ -16,8 +16,10 @@ struct Geo {
     int abcn;
     int W;                              // boundary strip width M + 1
     int nxp, nyp, nzp;                  // physical box extents
-    int64_t sxy;                        // nx * ny (z stride)
-    int64_t n;                          // cells per shot
+    int px;                             // row pitch of the core's arrays (nx rounded up to 32)
+    int64_t sxy;                        // px * ny (z stride)
+    int64_t n;                          // cells per shot in the core's arrays (nz * sxy)
+    int64_t nm;                         // cells per shot in the caller's arrays (nz * ny * nx)
     int64_t mstride;                    // model stride per shot (0: shared model)
     int64_t ns;                         // strip cells per shot per step (six faces)
     float dt, dx, dy, dz;

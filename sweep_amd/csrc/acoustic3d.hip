@@ -50,8 +50,11 @@ Geo make_geo(const swp_problem &p)
     g.nxp = g.X1 - g.X0;
     g.nyp = g.Y1 - g.Y0;
     g.nzp = g.Z1 - g.Z0;
-    g.sxy = int64_t(g.nx) * g.ny;
+    // rows of every array the core allocates start on 128-byte lines
+    g.px = (g.nx + 31) / 32 * 32;
+    g.sxy = int64_t(g.px) * g.ny;
     g.n = g.sxy * g.nz;
+    g.nm = int64_t(g.nz) * g.ny * g.nx;
     g.mstride = p.model_batched ? g.n : 0;
     const int64_t W = g.W;
     g.ns = 2 * W * g.nyp * g.nxp + 2 * W * g.nzp * g.nxp + 2 * W * g.nzp * g.nyp;
@@ -90,12 +93,12 @@ __global__ void __launch_bounds__(256) fwd_kernel(Geo g, Coef c, Prof pr, StepFi
     const int b = int(blockIdx.z) / nzb;
     if (ix >= bx.x1 || iy >= bx.y1) return;
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = (int64_t(iz) * g.ny + iy) * g.nx + ix;
+    const int64_t i = (int64_t(iz) * g.ny + iy) * g.px + ix;
     const float *__restrict__ un = f.u_now + off;
     const float *__restrict__ vp = f.vp + int64_t(b) * g.mstride;
     if (strip) save_strips(g, strip + int64_t(b) * g.ns, ix, iy, iz, un[i]);
 
-    const int64_t S[3] = {1, g.nx, g.sxy};
+    const int64_t S[3] = {1, g.px, g.sxy};
     const int pos[3] = {ix, iy, iz};
     const int lo[3] = {g.X0, g.Y0, g.Z0}, hi[3] = {g.X1, g.Y1, g.Z1};
     const float ih2[3] = {g.inv_dx2, g.inv_dy2, g.inv_dz2}, ih[3] = {g.inv_dx, g.inv_dy, g.inv_dz};
@@ -159,7 +162,7 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
     const int b = int(blockIdx.z) / nzb;
     if (ix >= bx.x1 || iy >= bx.y1) return;
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = (int64_t(iz) * g.ny + iy) * g.nx + ix;
+    const int64_t i = (int64_t(iz) * g.ny + iy) * g.px + ix;
     const float *__restrict__ ln = f.l_now + off;
     const float *__restrict__ vp = f.vp + int64_t(b) * g.mstride;
     const float dt2 = g.dt2;
@@ -169,7 +172,7 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
     const float v0 = vp[i];
     const float gw = v0 * v0 * dt2 * gun;
     auto GW = [&](int64_t nn) { return vp[nn] * vp[nn] * dt2 * ln[nn]; };
-    const int64_t S[3] = {1, g.nx, g.sxy};
+    const int64_t S[3] = {1, g.px, g.sxy};
     const int pos[3] = {ix, iy, iz};
     const int lo[3] = {g.X0, g.Y0, g.Z0}, hi[3] = {g.X1, g.Y1, g.Z1};
     const float ih2[3] = {g.inv_dx2, g.inv_dy2, g.inv_dz2}, ih[3] = {g.inv_dx, g.inv_dy, g.inv_dz};
@@ -255,11 +258,11 @@ __global__ void __launch_bounds__(256) recon_kernel(Geo g, Coef c, const float *
     const int b = int(blockIdx.z) / nzb;
     if (ix >= bx.x1 || iy >= bx.y1) return;
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = (int64_t(iz) * g.ny + iy) * g.nx + ix;
+    const int64_t i = (int64_t(iz) * g.ny + iy) * g.px + ix;
     const float *__restrict__ un = u_now + off;
     const float *__restrict__ vp = vpb + int64_t(b) * g.mstride;
     const float w_sum = fd::lap<MS>(c, [&](int k) { return un[i + k]; }, g.inv_dx2, M) +
-                        fd::lap<MS>(c, [&](int k) { return un[i + k * int64_t(g.nx)]; }, g.inv_dy2, M) +
+                        fd::lap<MS>(c, [&](int k) { return un[i + k * int64_t(g.px)]; }, g.inv_dy2, M) +
                         fd::lap<MS>(c, [&](int k) { return un[i + k * g.sxy]; }, g.inv_dz2, M);
     const float v = vp[i];
     const float u0 = un[i];
@@ -321,7 +324,7 @@ __global__ void __launch_bounds__(256) strip_restore_kernel(
         if (l[0] == ix && l[1] == iy && l[2] == iz) val += -src[(int64_t(b) * nsrc + k) * g.nt + it];
     }
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = (int64_t(iz) * g.ny + iy) * g.nx + ix;
+    const int64_t i = (int64_t(iz) * g.ny + iy) * g.px + ix;
     u_next[off + i] = val;
     if (grad) {
         const float u_tt = (val - 2 * u_now[off + i] + u_prev[off + i]) * g.inv_dt2;
@@ -344,7 +347,7 @@ __global__ void __launch_bounds__(256) image_kernel(Geo g, const float *__restri
     const int b = int(blockIdx.z) / g.nzp;
     if (ix >= g.X1 || iy >= g.Y1) return;
     const int64_t off = int64_t(b) * g.n;
-    const int64_t i = (int64_t(iz) * g.ny + iy) * g.nx + ix;
+    const int64_t i = (int64_t(iz) * g.ny + iy) * g.px + ix;
     const float l = lam[off + i];
     if (grad) grad[off + i] += 2.f * g.dt * g.dt * img[off + i] * l / vpb[int64_t(b) * g.mstride + i];
     if (R) R[off + i] += l * l;
@@ -536,6 +539,37 @@ static Tape carve_tape(Carver &cv, const swp_problem &p, const Geo &g)
     return t;
 }
 
+// The core's row-pitched copies of the caller's model and of the gradient
+// and illumination outputs (the caller's arrays are (B|1, nz, ny, nx) dense).
+struct IoBufs {
+    float *vp = nullptr;                                 // (B or 1, n)
+    float *grad = nullptr, *S = nullptr, *R = nullptr;   // (B, n), backward only (S, R with illumination)
+};
+
+static IoBufs carve_io(Carver &cv, const swp_problem &p, const Geo &g, int kind)
+{
+    IoBufs b;
+    b.vp = cv.take<float>((p.model_batched ? int64_t(g.B) : 1) * g.n);
+    if (kind == 1) {
+        const int64_t N = int64_t(g.B) * g.n;
+        b.grad = cv.take<float>(N);
+        if (p.flags & 1) {
+            b.S = cv.take<float>(N);
+            b.R = cv.take<float>(N);
+        }
+    }
+    return b;
+}
+
+// q = p with the model pointer on the core's pitched copy (filled here).
+static swp_problem with_pitched_model(const swp_problem &p, const Geo &g, const IoBufs &b)
+{
+    swp_problem q = p;
+    pitch_copy(b.vp, g.px, p.models[0], g.nx, (p.model_batched ? int64_t(g.B) : 1) * g.nz * g.ny, g.nx);
+    q.models[0] = b.vp;
+    return q;
+}
+
 static PointArgs point_args(const Geo &g)
 {
     PointArgs a{};
@@ -544,6 +578,7 @@ static PointArgs point_args(const Geo &g)
     a.nz = g.nz;
     a.ny = g.ny;
     a.nx = g.nx;
+    a.px = g.px;
     a.n = g.n;
     a.sign = 1.0f;
     return a;
@@ -622,8 +657,9 @@ int64_t workspace_bytes(const swp_problem &p, int kind)
     if (kind == 1) {
         AdjState as;
         carve_adj(cv, g, as);
-        if (p.memory == SWP_MEM_CKPT) cv.take<float>(int64_t(p.ckpt_interval) * g.B * g.n);
     }
+    carve_io(cv, p, g, kind);
+    if (kind == 1 && p.memory == SWP_MEM_CKPT) cv.take<float>(int64_t(p.ckpt_interval) * g.B * g.n);
     return cv.used + 256;
 }
 
@@ -635,14 +671,15 @@ int64_t tape_bytes(const swp_problem &p)
     return cv.used + 256;
 }
 
-void forward(const swp_problem &p, float *record)
+void forward(const swp_problem &p_in, float *record)
 {
-    const Ctx x = make_ctx(p);
+    const Ctx x = make_ctx(p_in);
     const Geo &g = x.g;
-    Carver ws(p.workspace, p.workspace_bytes);
+    Carver ws(p_in.workspace, p_in.workspace_bytes);
     FwdState s;
     carve_fwd(ws, g, s);
     zero_fwd(g, s);
+    const swp_problem p = with_pitched_model(p_in, g, carve_io(ws, p_in, g, 0));
     Carver tc(p.tape, p.tape_bytes);
     Tape t = carve_tape(tc, p, g);
     const int64_t N = int64_t(g.B) * g.n;
@@ -725,17 +762,40 @@ static StripW strip_widths(const Geo &g)
     return w;
 }
 
-void backward(const swp_problem &p, const swp_backward_io &io)
+static void backward_run(const Ctx &x, const swp_problem &p, const swp_backward_io &io, Carver &ws,
+                         FwdState &fs, AdjState &a);
+
+void backward(const swp_problem &p_in, const swp_backward_io &io_in)
 {
-    const Ctx x = make_ctx(p);
+    const Ctx x = make_ctx(p_in);
     const Geo &g = x.g;
-    const int64_t N = int64_t(g.B) * g.n;
-    Carver ws(p.workspace, p.workspace_bytes);
+    Carver ws(p_in.workspace, p_in.workspace_bytes);
     FwdState fs;
     carve_fwd(ws, g, fs);
     AdjState a;
     carve_adj(ws, g, a);
     zero_adj(g, a);
+    const IoBufs b = carve_io(ws, p_in, g, 1);
+    const swp_problem p = with_pitched_model(p_in, g, b);
+    // gradients and illumination accumulate in the core's pitched buffers,
+    // then add into the caller's
+    swp_backward_io io = io_in;
+    const int64_t N = int64_t(g.B) * g.n, rows = int64_t(g.B) * g.nz * g.ny;
+    const bool illum = (p.flags & 1) != 0;
+    if (io_in.grad_models[0]) { fill_zero(b.grad, N); io.grad_models[0] = b.grad; }
+    if (illum && io_in.illum_source) { fill_zero(b.S, N); io.illum_source = b.S; }
+    if (illum && io_in.illum_receiver) { fill_zero(b.R, N); io.illum_receiver = b.R; }
+    backward_run(x, p, io, ws, fs, a);
+    if (io_in.grad_models[0]) pitch_add(io_in.grad_models[0], g.nx, b.grad, g.px, rows, g.nx);
+    if (illum && io_in.illum_source) pitch_add(io_in.illum_source, g.nx, b.S, g.px, rows, g.nx);
+    if (illum && io_in.illum_receiver) pitch_add(io_in.illum_receiver, g.nx, b.R, g.px, rows, g.nx);
+}
+
+static void backward_run(const Ctx &x, const swp_problem &p, const swp_backward_io &io, Carver &ws,
+                         FwdState &fs, AdjState &a)
+{
+    const Geo &g = x.g;
+    const int64_t N = int64_t(g.B) * g.n;
     Carver tc(p.tape, p.tape_bytes);
     Tape t = carve_tape(tc, p, g);
     float *grad = io.grad_models[0];

@@ -201,7 +201,7 @@ struct XHalo {
     uint32_t off;   // in-plane byte offset (clamped into the grid)
     int col;        // global column (clamped)
     int lds;        // index in the LDS plane, -1: no duty
-    __device__ XHalo(int lane, int w, int xo, int yo, int nx, int ny)
+    __device__ XHalo(int lane, int w, int xo, int yo, int nx, int px, int ny)
     {
         if (lane < 2 * M * RY) {
             const int r = lane / (2 * M), c = lane % (2 * M);
@@ -209,7 +209,7 @@ struct XHalo {
             const int gx = c < M ? xo - M + c : xo + 64 + (c - M);
             const int gy = min(yo + row, ny - 1);
             col = min(max(gx, 0), nx - 1);
-            off = uint32_t(gy * nx + col) * 4u;
+            off = uint32_t(gy * px + col) * 4u;
             lds = (row + M) * (64 + 2 * M) + (c < M ? c : 64 + M + (c - M));
         } else {
             col = 0;
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
     const int64_t sxy = g.sxy;
     const Lanes<M, RY> L(Rc, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
-    const XHalo<M, RY> xh(lane, w, xo, yo, nx, ny);
+    const XHalo<M, RY> xh(lane, w, xo, yo, nx, g.px, ny);
     const int64_t off = int64_t(b) * g.n;
     const float dt = g.dt;
     // fields from plane zb - M on; plane z sits at byte offset pl(z)
@@ -272,9 +272,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
 
     uint32_t col[RY], yhc[YH];   // in-plane byte offsets
 #pragma unroll
-    for (int r = 0; r < RY; ++r) col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
+    for (int r = 0; r < RY; ++r) col[r] = uint32_t(L.yc[r] * g.px + L.xc) * 4u;
 #pragma unroll
-    for (int t = 0; t < YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
+    for (int t = 0; t < YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * g.px + L.xc) * 4u;
 
     float ax = 0.f, bx = 0.f, dbx = 0.f, dax = 0.f;
     bool inbx = false;
@@ -524,7 +524,7 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
     const int64_t sxy = g.sxy;
     const Lanes<M, RY> L(Rc, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
-    const XHalo<M, RY> xh(lane, w, xo, yo, nx, ny);
+    const XHalo<M, RY> xh(lane, w, xo, yo, nx, g.px, ny);
     const int64_t off = int64_t(b) * g.n;
     const float dt2 = g.dt2;
     // fields from plane zb - M on; plane z sits at byte offset pl(z)
@@ -541,9 +541,9 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
 
     uint32_t col[RY], yhc[YH];   // in-plane byte offsets
 #pragma unroll
-    for (int r = 0; r < RY; ++r) col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
+    for (int r = 0; r < RY; ++r) col[r] = uint32_t(L.yc[r] * g.px + L.xc) * 4u;
 #pragma unroll
-    for (int t = 0; t < YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
+    for (int t = 0; t < YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * g.px + L.xc) * 4u;
 
     // profiles: the lane's column / x-halo duty column (x), rows / halo rows (y)
     float ax = 0.f, bx = 0.f, dbx = 0.f, dax = 0.f, hax = 0.f, hbx = 0.f, hdbx = 0.f;
@@ -900,12 +900,12 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
     uint32_t col[RY], xhc[RY], yhc[PM::YH];   // in-plane byte offsets
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-        col[r] = uint32_t(L.yc[r] * nx + L.xc) * 4u;
-        xhc[r] = uint32_t(L.yc[r] * nx + pm.xh_col) * 4u;
+        col[r] = uint32_t(L.yc[r] * g.px + L.xc) * 4u;
+        xhc[r] = uint32_t(L.yc[r] * g.px + pm.xh_col) * 4u;
         img_y[r] = L.y[r] >= g.Y0 + g.W && L.y[r] < g.Y1 - g.W;
     }
 #pragma unroll
-    for (int t = 0; t < PM::YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * nx + L.xc) * 4u;
+    for (int t = 0; t < PM::YH; ++t) yhc[t] = uint32_t((pm.yh_row[t] >= 0 ? pm.yh_row[t] : 0) * g.px + L.xc) * 4u;
 
     constexpr int Q = 2 * M + 1;
     float q[RY][Q];
@@ -1000,11 +1000,13 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
 // ---------------------------------------------------------------- host plans
 constexpr int kRyInt = 4;     // rows per lane of the interior kernels (tile 64 x 16)
 
-ZRect make_rect(int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, int nx)
+ZRect make_rect(int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, int px)
 {
     ZRect R{};
     R.x0 = x0; R.x1 = x1; R.y0 = y0; R.y1 = y1; R.z0 = z0; R.z1 = z1; R.tz = tz;
-    R.xs = nx % 32 == 0 ? x0 - x0 % 32 : x0;
+    // line-aligned tile origin unless that costs an extra column of tiles
+    const int xa = px % 32 == 0 ? x0 - x0 % 32 : x0;
+    R.xs = (x1 - xa + 63) / 64 <= (x1 - x0 + 63) / 64 ? xa : x0;
     R.ntx = (x1 - R.xs + 63) / 64;
     R.nty = (y1 - y0 + TY - 1) / TY;
     R.nseg = (z1 - z0 + tz - 1) / tz;
@@ -1012,11 +1014,11 @@ ZRect make_rect(int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, 
     return R;
 }
 
-void add_rect(ZSet &s, int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, int nx)
+void add_rect(ZSet &s, int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, int px)
 {
     if (x1 <= x0 || y1 <= y0 || z1 <= z0) return;
     SWP_CHECK(s.n < kRectMax, "acoustic3d: too many block rectangles");
-    s.r[s.n] = make_rect(x0, x1, y0, y1, z0, z1, tz, TY, nx);
+    s.r[s.n] = make_rect(x0, x1, y0, y1, z0, z1, tz, TY, px);
     s.per_shot += s.r[s.n].nblk;
     ++s.n;
 }
@@ -1029,20 +1031,24 @@ struct Cuts {
     int c0, a, b, c1;
 };
 
-Cuts cuts(int n, int M, int lo, int hi)
+// align > 1 (the x axis on line-aligned rows): the interior starts at the
+// next multiple of align, so its tiles cover whole 128-byte lines; the low
+// margin takes the few extra cells (its kernels run them without CPML terms).
+Cuts cuts(int n, int M, int lo, int hi, int align = 1)
 {
     Cuts k;
     k.c0 = M;
     k.c1 = n - M;
     k.a = std::min(std::max(lo + M + 1, k.c0), k.c1);
     k.b = std::min(std::max(hi - M - 1, k.a), k.c1);
+    if (align > 1) k.a = std::min((k.a + align - 1) / align * align, k.b);
     return k;
 }
 
 // The regions whose axes lie in a margin exactly where (ex, ey, ez) say so.
 ZSet region_set(const Geo &g, bool ex, bool ey, bool ez, int tz, int TY)
 {
-    const Cuts X = cuts(g.nx, g.M, g.X0, g.X1), Y = cuts(g.ny, g.M, g.Y0, g.Y1), Z = cuts(g.nz, g.M, g.Z0, g.Z1);
+    const Cuts X = cuts(g.nx, g.M, g.X0, g.X1, 32), Y = cuts(g.ny, g.M, g.Y0, g.Y1), Z = cuts(g.nz, g.M, g.Z0, g.Z1);
     int xs[2][2], ys[2][2], zs[2][2];
     int nxr = 0, nyr = 0, nzr = 0;
     auto fill = [](const Cuts &k, bool edge, int out[2][2], int &n) {
@@ -1062,7 +1068,7 @@ ZSet region_set(const Geo &g, bool ex, bool ey, bool ez, int tz, int TY)
     for (int i = 0; i < nzr; ++i)
         for (int j = 0; j < nyr; ++j)
             for (int l = 0; l < nxr; ++l)
-                add_rect(s, xs[l][0], xs[l][1], ys[j][0], ys[j][1], zs[i][0], zs[i][1], tz, TY, g.nx);
+                add_rect(s, xs[l][0], xs[l][1], ys[j][0], ys[j][1], zs[i][0], zs[i][1], tz, TY, g.px);
     return s;
 }
 
@@ -1128,7 +1134,7 @@ bool zm_supported(const Geo &g, int &tz)
 int shell_boxes(const Geo &g, Box out[6])
 {
     // the six slabs around the lean interior (per-cell shell launches)
-    const Cuts X = cuts(g.nx, g.M, g.X0, g.X1), Y = cuts(g.ny, g.M, g.Y0, g.Y1), Z = cuts(g.nz, g.M, g.Z0, g.Z1);
+    const Cuts X = cuts(g.nx, g.M, g.X0, g.X1, 32), Y = cuts(g.ny, g.M, g.Y0, g.Y1), Z = cuts(g.nz, g.M, g.Z0, g.Z1);
     const Box cand[6] = {
         {X.c0, X.c1, Y.c0, Y.c1, Z.c0, Z.a}, {X.c0, X.c1, Y.c0, Y.c1, Z.b, Z.c1},
         {X.c0, X.c1, Y.c0, Y.a, Z.a, Z.b},   {X.c0, X.c1, Y.b, Y.c1, Z.a, Z.b},
@@ -1181,7 +1187,7 @@ void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const f
 {
     const Geo &g = x.g;
     ZSet s{};
-    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyInt, x.g.nx);
+    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyInt, x.g.px);
     if (!s.per_shot) return;
     by_order(g.M, [&](auto ms) {
         constexpr int M = decltype(ms)::value;

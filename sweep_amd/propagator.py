@@ -112,6 +112,8 @@ class _Wrapper(torch.autograd.Function):
         if call.memory == _core.MEM_CKPT and getattr(call, "recursive", False):
             entry = 4
         name = call.names[entry]
+        if call.illumination:
+            P.flags |= 1   # before the query: the workspace depends on it
         ws, _ = _core.query(name, P)
         workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
         P.workspace, P.workspace_bytes = workspace.data_ptr(), ws
@@ -130,7 +132,6 @@ class _Wrapper(torch.autograd.Function):
             S = torch.zeros(grid, dtype=torch.float32, device=dev)
             R = torch.zeros(grid, dtype=torch.float32, device=dev)
             io.illum_source, io.illum_receiver = _ptr(S), _ptr(R)
-            P.flags |= 1
         P.tape, P.tape_bytes = ctx.tape.data_ptr(), ctx.tape.numel()
         call.eq._C()[entry](P, io, dev)
         ctx.tape = None
