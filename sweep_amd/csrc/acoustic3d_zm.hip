@@ -1095,7 +1095,7 @@ dim3 grid_of(const ZSet &s, int B)
 template <bool BX, bool BY, bool BZ>
 constexpr int ry_fwd() { return (int(BX) + int(BY) + int(BZ)) <= 1 ? 2 : 1; }
 template <bool BX, bool BY, bool BZ>
-constexpr int ry_adj() { return 1; }
+constexpr int ry_adj() { return (int(BX) + int(BY) + int(BZ)) <= 1 ? 2 : 1; }
 
 template <int M, bool BX, bool BY, bool BZ>
 void fwd_region(const Ctx &x, const StepFields &f, float *u_this, float *strip)
