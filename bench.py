@@ -201,6 +201,28 @@ def cpu_baseline(W, sample_nt):
                       f"on the full {grid} runtime grid ({dt:.1f} s), extrapolated x{scale:g}"}
 
 
+# ----------------------------------------------------------------- traffic
+# HBM bytes per launch of a kernel class, measured by the two rocprofv3 PMC
+# passes recorded in profiles/ (FETCH_SIZE scaled by 2.000 and WRITE_SIZE by
+# 1.000, the factors a 1 GiB device copy reads in the same runs), for the
+# default shots-per-launch of the workload.  Counters cannot be read inside
+# this process, so the figure is the recorded one.
+MEASURED_TRAFFIC = {
+    # profiles/acoustic2d_384x1152_32shots_bs.md: adj_band 359.9 MB + adj_int 258.2 MB
+    ("acoustic2d", 1): 618.1e6,
+    # profiles/acoustic3d_256x512x512_8shots_bs.md: the eight adj3_kernel regions of one step, 2 shots
+    ("acoustic3d", 1): 10549.3e6,
+    # profiles/elastic2d_512x1024_16shots_ckpt.md: vel_kernel 823.7 MB + stress_kernel 1010.6 MB
+    ("elastic2d", 0): 1834.3e6,
+}
+
+
+def measured_traffic(workload, cls, per_launch):
+    if per_launch != WORKLOADS[workload]["per_launch"]:
+        return None
+    return MEASURED_TRAFFIC.get((workload, cls))
+
+
 # ----------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -213,6 +235,9 @@ def main():
     ap.add_argument("--cpu-sample-nt", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None, help="override nt (profiling runs only)")
     ap.add_argument("--shots-per-launch", type=int, default=None)
+    ap.add_argument("--pmc-launch", action="store_true",
+                    help="counter runs: a 1 GiB device copy (the calibration of FETCH_SIZE / WRITE_SIZE), "
+                         "then one untimed gradient launch; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -226,6 +251,19 @@ def main():
     dev = torch.device("cuda:0")
     solver, vp_true, vp_init, src, rec, mask, wav = make_problem(W, dev, nt)
     nshots = src.shape[0]
+    if args.pmc_launch:
+        ca = torch.ones(1 << 28, device=dev)
+        cb = torch.empty_like(ca)
+        cb.copy_(ca)
+        torch.cuda.synchronize()
+        del ca, cb
+        ms = [torch.tensor(m, device=dev, requires_grad=True) for m in vp_init]
+        pred = solver(wav, src[:per_launch], rec[:per_launch], models=ms)
+        (0.5 * pred.pow(2).sum()).backward()
+        torch.cuda.synchronize()
+        print(f"[bench] {W['name']} nt={nt}: one launch of {min(per_launch, nshots)} shots done", file=sys.stderr,
+              flush=True)
+        return
     batches = [(b, min(b + per_launch, nshots)) for b in range(0, nshots, per_launch)]
     print(f"[bench] {W['name']} nt={nt}: building obs", file=sys.stderr, flush=True)
     vt = [torch.tensor(m, device=dev) for m in vp_true]
@@ -287,7 +325,8 @@ def main():
                    "shots_per_launch": per_launch, "order": W["order"],
                    "memory": "checkpoint" if elastic else "boundary_saving"},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, dom, per_launch),
                      "algorithmic_bytes_per_launch": int(abytes[dom]),
                      "avg_launch_ms": per[dom]["avg_ms"],
                      "kernels": {names[c]: {"avg_launch_ms": per[c]["avg_ms"], "GB/s": per[c]["gbs"],
@@ -296,7 +335,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
-        sample = args.cpu_sample_nt or (4 if len(W["shape"]) == 3 else (400 if elastic else 1000))
+        sample = args.cpu_sample_nt or (8 if len(W["shape"]) == 3 else (3000 if elastic else W["nt"]))
         out["cpu_baseline"] = cpu_baseline(W, sample)
     print(json.dumps(out), flush=True)
 
