@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWP_ABI_VERSION 2
+#define SWP_ABI_VERSION 3
 
 /* Memory strategy of a gradient run (reference: propagator/options.py
  * Full / BoundarySaving(storage='gpu') / Ckpt(mode='chunk'|'recursive')). */
@@ -37,6 +37,8 @@ enum {
     SWP_MEM_FULL = 1,      /* per-step imaging field (nt, B, grid) */
     SWP_MEM_BOUNDARY = 2,  /* boundary strips + the last two states */
     SWP_MEM_CKPT = 3,      /* state snapshots every ckpt_interval steps, chunk replay */
+    SWP_MEM_BINOMIAL = 4,  /* ckpt_interval state snapshot slots, binomial reversal
+                              (served by the *_backward_recursive_ckpt entries) */
 };
 
 /* One propagation problem on the runtime grid.  Arrays are row-major with x
@@ -141,6 +143,17 @@ SWP_CAPI void *swp_get_stream(void);
  * reference counterpart: the roofline hook of bench.py. */
 SWP_CAPI void swp_set_timing(int stride);
 SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launches);
+
+/* The binomial checkpoint schedule SWP_MEM_BINOMIAL runs, for inspection
+ * and host-side tests: writes min(count, cap) (code, arg) int32 pairs --
+ * code 0 step arg, 1 save to slot arg, 2 load slot arg, 3 re-run and reverse
+ * step arg -- and returns count, or -1 on bad arguments.  swp_revolve_spine
+ * returns the (slot, step) snapshots the forward pass takes.  Replaces the
+ * recursive-bisection schedule of the reference's recursive checkpoint
+ * backward (csrc/cuda/common/eq_driver.cuh:1055-1259, steps from
+ * propagator/_c.py:2608-2615); no C function of the reference corresponds. */
+SWP_CAPI int64_t swp_revolve_schedule(int32_t nt, int32_t slots, int32_t *ops, int64_t cap);
+SWP_CAPI int64_t swp_revolve_spine(int32_t nt, int32_t slots, int32_t *pts, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -15,12 +15,12 @@ import threading
 from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libswp_core.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.POINTER(ctypes.c_int32)
 
-MEM_NONE, MEM_FULL, MEM_BOUNDARY, MEM_CKPT = 0, 1, 2, 3
+MEM_NONE, MEM_FULL, MEM_BOUNDARY, MEM_CKPT, MEM_BINOMIAL = 0, 1, 2, 3, 4
 
 
 class SwpProblem(ctypes.Structure):
@@ -68,6 +68,7 @@ EXPORTS = (
     "swp_core_abi_version", "swp_layout_count", "swp_layout_probe", "swp_entry_count",
     "swp_entry_name", "swp_entry_kind", "swp_entry_available", "swp_core_device", "swp_query",
     "swp_call", "swp_set_stream", "swp_get_stream", "swp_set_timing", "swp_last_kernel_stats",
+    "swp_revolve_schedule", "swp_revolve_spine",
 )
 
 
@@ -102,6 +103,9 @@ def _declare(L):
     L.swp_last_kernel_stats.restype = ctypes.c_int
     L.swp_last_kernel_stats.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int64)]
+    for f in (L.swp_revolve_schedule, L.swp_revolve_spine):
+        f.restype = ctypes.c_int64
+        f.argtypes = [ctypes.c_int32, ctypes.c_int32, _I32, ctypes.c_int64]
 
 
 def load():
@@ -187,3 +191,25 @@ def kernel_stats(which: int):
     if L.swp_last_kernel_stats(which, ctypes.byref(ms), ctypes.byref(n)) != 0:
         raise RuntimeError("swp_last_kernel_stats failed")
     return float(ms.value), int(n.value)
+
+
+def _pairs(fn, nt: int, slots: int):
+    import numpy as np
+    n = fn(int(nt), int(slots), None, 0)
+    if n < 0:
+        raise ValueError(f"bad checkpoint schedule arguments nt={nt}, slots={slots}")
+    out = np.zeros((max(n, 1), 2), dtype=np.int32)
+    fn(int(nt), int(slots), out.ctypes.data_as(_I32), n)
+    return out[:n]
+
+
+def revolve_schedule(nt: int, slots: int):
+    """The binomial checkpoint schedule of Ckpt(mode='recursive', count=slots)
+    as an (n, 2) int32 array of (code, arg): 0 step arg, 1 save to slot arg,
+    2 load slot arg, 3 re-run and reverse step arg.  Host only (no GPU)."""
+    return _pairs(load().swp_revolve_schedule, nt, slots)
+
+
+def revolve_spine(nt: int, slots: int):
+    """(slot, step) snapshots the forward pass of that schedule takes."""
+    return _pairs(load().swp_revolve_spine, nt, slots)

@@ -13,6 +13,7 @@
 // residual -> rotate -> wavelet-gradient gather.
 #include "acoustic3d.h"
 #include "acoustic3d_zm.h"
+#include "revolve.h"
 #include "fd.h"
 
 #include <algorithm>
@@ -535,6 +536,10 @@ static Tape carve_tape(Carver &cv, const swp_problem &p, const Geo &g)
         SWP_CHECK(p.ckpt_interval >= 1, "ckpt_interval must be >= 1");
         t.nchunk = (g.nt + p.ckpt_interval - 1) / p.ckpt_interval;
         t.snaps = cv.take<float>(int64_t(t.nchunk) * kSnapFields * N);
+    } else if (p.memory == SWP_MEM_BINOMIAL) {
+        SWP_CHECK(p.ckpt_interval >= 1, "binomial checkpointing needs >= 1 snapshot slot");
+        t.nchunk = p.ckpt_interval;
+        t.snaps = cv.take<float>(int64_t(t.nchunk) * kSnapFields * N);
     }
     return t;
 }
@@ -660,6 +665,7 @@ int64_t workspace_bytes(const swp_problem &p, int kind)
     }
     carve_io(cv, p, g, kind);
     if (kind == 1 && p.memory == SWP_MEM_CKPT) cv.take<float>(int64_t(p.ckpt_interval) * g.B * g.n);
+    if (kind == 1 && p.memory == SWP_MEM_BINOMIAL) cv.take<float>(int64_t(g.B) * g.n);
     return cv.used + 256;
 }
 
@@ -683,9 +689,14 @@ void forward(const swp_problem &p_in, float *record)
     Carver tc(p.tape, p.tape_bytes);
     Tape t = carve_tape(tc, p, g);
     const int64_t N = int64_t(g.B) * g.n;
+    std::vector<SpinePt> spine;
+    if (p.memory == SWP_MEM_BINOMIAL) spine = revolve_spine(g.nt, t.nchunk);
+    size_t sp = 0;
     for (int it = 0; it < g.nt; ++it) {
-        if (t.snaps && it % p.ckpt_interval == 0)
+        if (p.memory == SWP_MEM_CKPT && it % p.ckpt_interval == 0)
             snapshot(g, s, t.snaps + int64_t(it / p.ckpt_interval) * kSnapFields * N);
+        for (; sp < spine.size() && spine[sp].step == it; ++sp)
+            snapshot(g, s, t.snaps + int64_t(spine[sp].slot) * kSnapFields * N);
         forward_step(x, p, s, it, t.u_this ? t.u_this + int64_t(it) * N : nullptr,
                      t.strips ? t.strips + int64_t(it) * g.B * g.ns : nullptr, record);
     }
@@ -810,6 +821,26 @@ static void backward_run(const Ctx &x, const swp_problem &p, const swp_backward_
             if (S || R) image(x, p, t.u_this + int64_t(it) * N, nullptr, nullptr, a.l[1], nullptr, S, R);
         }
         if (grad && g.nt > 0) image(x, p, t.u_this, nullptr, nullptr, a.l[1], grad, nullptr, nullptr);
+        return;
+    }
+
+    if (p.memory == SWP_MEM_BINOMIAL) {
+        // the forward left the spine snapshots; the schedule re-runs steps
+        // from them and reverses each step right after re-running it
+        float *cap = ws.take<float>(N);
+        zero_fwd(g, fs);   // the cells no step writes (rim, psi outside the bands) stay zero
+        for (const RevOp &o : revolve_schedule(g.nt, t.nchunk)) {
+            switch (o.code) {
+            case REV_STEP: forward_step(x, p, fs, o.arg, nullptr, nullptr, nullptr); break;
+            case REV_SAVE: snapshot(g, fs, t.snaps + int64_t(o.arg) * kSnapFields * N); break;
+            case REV_LOAD: restore_snapshot(g, fs, t.snaps + int64_t(o.arg) * kSnapFields * N); break;
+            default:
+                forward_step(x, p, fs, o.arg, cap, nullptr, nullptr);
+                adjoint_step(x, p, io, a, o.arg, nullptr, nullptr);
+                image(x, p, cap, nullptr, nullptr, a.l[1], grad, S, R);
+                break;
+            }
+        }
         return;
     }
 

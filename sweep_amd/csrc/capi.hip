@@ -2,6 +2,7 @@
 // dispatch, stream, layout probe, timing.  Every function catches everything
 // and answers with a return code plus a message, like the reference's
 // guarded() (csrc/cuda/common/capi.cu:230-242).
+#include <algorithm>
 #include <cstddef>
 #include <cstring>
 #include <exception>
@@ -10,6 +11,7 @@
 #include "acoustic2d.h"
 #include "acoustic3d.h"
 #include "elastic2d.h"
+#include "revolve.h"
 
 namespace swp {
 
@@ -29,17 +31,17 @@ const Entry kEntries[] = {
     {"acoustic2d_backward", 1, 0, SWP_MEM_FULL},
     {"acoustic2d_backward_bs", 1, 0, SWP_MEM_BOUNDARY},
     {"acoustic2d_backward_ckpt", 1, 0, SWP_MEM_CKPT},
-    {"acoustic2d_backward_recursive_ckpt", 1, 0, SWP_MEM_CKPT},
+    {"acoustic2d_backward_recursive_ckpt", 1, 0, SWP_MEM_BINOMIAL},
     {"acoustic3d_forward", 0, 1, 0},
     {"acoustic3d_backward", 1, 1, SWP_MEM_FULL},
     {"acoustic3d_backward_bs", 1, 1, SWP_MEM_BOUNDARY},
     {"acoustic3d_backward_ckpt", 1, 1, SWP_MEM_CKPT},
-    {"acoustic3d_backward_recursive_ckpt", 1, 1, SWP_MEM_CKPT},
+    {"acoustic3d_backward_recursive_ckpt", 1, 1, SWP_MEM_BINOMIAL},
     {"elastic2d_forward", 0, 2, 0},
     {"elastic2d_backward", 1, 2, SWP_MEM_FULL},
     {"elastic2d_backward_bs", 1, 2, SWP_MEM_BOUNDARY},
     {"elastic2d_backward_ckpt", 1, 2, SWP_MEM_CKPT},
-    {"elastic2d_backward_recursive_ckpt", 1, 2, SWP_MEM_CKPT},
+    {"elastic2d_backward_recursive_ckpt", 1, 2, SWP_MEM_BINOMIAL},
 };
 constexpr int kNumEntries = int(sizeof(kEntries) / sizeof(kEntries[0]));
 
@@ -228,6 +230,36 @@ SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launche
         if (ms_total) *ms_total = ms;
         if (launches) *launches = n;
     });
+}
+
+SWP_CAPI int64_t swp_revolve_schedule(int32_t nt, int32_t slots, int32_t *ops, int64_t cap)
+{
+    if (nt < 0 || slots < 1) return -1;
+    try {
+        const std::vector<RevOp> v = revolve_schedule(nt, slots);
+        for (int64_t k = 0; ops && k < std::min<int64_t>(cap, int64_t(v.size())); ++k) {
+            ops[2 * k] = v[size_t(k)].code;
+            ops[2 * k + 1] = v[size_t(k)].arg;
+        }
+        return int64_t(v.size());
+    } catch (...) {
+        return -1;
+    }
+}
+
+SWP_CAPI int64_t swp_revolve_spine(int32_t nt, int32_t slots, int32_t *pts, int64_t cap)
+{
+    if (nt < 0 || slots < 1) return -1;
+    try {
+        const std::vector<SpinePt> v = revolve_spine(nt, slots);
+        for (int64_t k = 0; pts && k < std::min<int64_t>(cap, int64_t(v.size())); ++k) {
+            pts[2 * k] = v[size_t(k)].slot;
+            pts[2 * k + 1] = v[size_t(k)].step;
+        }
+        return int64_t(v.size());
+    } catch (...) {
+        return -1;
+    }
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 // 182-210; common/derived_models.cu:27-37) without the free-surface
 // branches; the time loops follow common/sg_driver.cuh:23-110, 285-340.
 #include "elastic2d.h"
+#include "revolve.h"
 
 #include <algorithm>
 
@@ -429,8 +430,8 @@ struct Work {
     Fields a;      // adjoint state
     Fields qp;     // q (0..3), p (4..7)
     float *zero;   // (2, B, n) zero velocities: v(nt)
-    float *carry;  // (2, B, n) v(chunk end) for the checkpoint reverse
-    float *chunk;  // (I, 2, B, n) replayed velocities
+    float *carry;  // (2, B, n) v(chunk end) for the checkpoint reverse; v(it + 1) for the binomial one
+    float *chunk;  // (I, 2, B, n) replayed velocities; (2, B, n) v(it) for the binomial reverse
 };
 
 int64_t model_cells(const Geo &g) { return g.mstride ? int64_t(g.B) * g.n : g.n; }
@@ -448,6 +449,9 @@ void carve(Carver &cv, const swp_problem &p, const Geo &g, int kind, Work &w)
     if (p.memory == SWP_MEM_CKPT) {
         w.carry = cv.take<float>(2 * N);
         w.chunk = cv.take<float>(int64_t(p.ckpt_interval) * 2 * N);
+    } else if (p.memory == SWP_MEM_BINOMIAL) {
+        w.carry = cv.take<float>(2 * N);
+        w.chunk = cv.take<float>(2 * N);
     }
 }
 
@@ -468,6 +472,10 @@ Tape carve_tape(Carver &cv, const swp_problem &p, const Geo &g)
     } else if (p.memory == SWP_MEM_CKPT) {
         SWP_CHECK(p.ckpt_interval >= 1, "ckpt_interval must be >= 1");
         t.nchunk = (g.nt + p.ckpt_interval - 1) / p.ckpt_interval;
+        t.snaps = cv.take<float>(int64_t(t.nchunk) * kSnap * N);
+    } else if (p.memory == SWP_MEM_BINOMIAL) {
+        SWP_CHECK(p.ckpt_interval >= 1, "binomial checkpointing needs >= 1 snapshot slot");
+        t.nchunk = p.ckpt_interval;
         t.snaps = cv.take<float>(int64_t(t.nchunk) * kSnap * N);
     }
     return t;
@@ -644,8 +652,14 @@ void forward(const swp_problem &p, float *record)
     Carver tc(p.tape, p.tape_bytes);
     Tape t = carve_tape(tc, p, g);
     const int64_t N = int64_t(g.B) * g.n;
+    std::vector<SpinePt> spine;
+    if (p.memory == SWP_MEM_BINOMIAL) spine = revolve_spine(g.nt, t.nchunk);
+    size_t sp = 0;
     for (int it = 0; it < g.nt; ++it) {
-        if (t.snaps && it % p.ckpt_interval == 0) snapshot(g, w.s, t.snaps + int64_t(it / p.ckpt_interval) * kSnap * N);
+        if (p.memory == SWP_MEM_CKPT && it % p.ckpt_interval == 0)
+            snapshot(g, w.s, t.snaps + int64_t(it / p.ckpt_interval) * kSnap * N);
+        for (; sp < spine.size() && spine[sp].step == it; ++sp)
+            snapshot(g, w.s, t.snaps + int64_t(spine[sp].slot) * kSnap * N);
         forward_step(x, p, w, it, t.vel ? t.vel + int64_t(it) * 2 * N : nullptr, record);
     }
 }
@@ -670,6 +684,25 @@ void backward(const swp_problem &p, const swp_backward_io &io)
             const float *vnow = t.vel + int64_t(it) * 2 * N;
             const float *vnext = it + 1 < g.nt ? t.vel + int64_t(it + 1) * 2 * N : w.zero;
             reverse_step(x, p, io, w, it, vnow, vnext);
+        }
+        return;
+    }
+    if (p.memory == SWP_MEM_BINOMIAL) {
+        // the forward left the spine snapshots; each reverse step re-runs its
+        // forward step for v(it), v(it + 1) being the previous reverse step's
+        fill_zero(w.carry, 2 * N);   // v(nt) = 0
+        zero_state(g, w.s);
+        for (const RevOp &o : revolve_schedule(g.nt, t.nchunk)) {
+            switch (o.code) {
+            case REV_STEP: forward_step(x, p, w, o.arg, nullptr, nullptr); break;
+            case REV_SAVE: snapshot(g, w.s, t.snaps + int64_t(o.arg) * kSnap * N); break;
+            case REV_LOAD: restore(g, w.s, t.snaps + int64_t(o.arg) * kSnap * N); break;
+            default:
+                forward_step(x, p, w, o.arg, w.chunk, nullptr);
+                reverse_step(x, p, io, w, o.arg, w.chunk, w.carry);
+                std::swap(w.chunk, w.carry);
+                break;
+            }
         }
         return;
     }

@@ -9,7 +9,9 @@ names, fields and validation, for the parts this build implements.
   forward field backwards in time (the ``impl='c'`` default).
 * ``Ckpt(mode='chunk', chunks=c)`` keeps ``c`` state snapshots and replays
   each chunk before reversing it.  ``mode='recursive'`` with ``count`` slots
-  uses the same chunk replay with ``count`` snapshots.
+  keeps ``count`` snapshots and nothing else, and reverses on a binomial
+  schedule: with r the smallest integer with C(count + r, count) >= nt, every
+  step is re-run at most r times (the reference bisects recursively).
 
 Host/disk staging, lossy or compressed strip storage and tail truncation are
 storage tiers outside this build; asking for them raises.

@@ -108,9 +108,7 @@ class _Wrapper(torch.autograd.Function):
         else:
             adj = grad[..., 0].permute(0, 2, 1).contiguous()
         adj = adj.to(torch.float32)
-        entry = {_core.MEM_FULL: 1, _core.MEM_BOUNDARY: 2, _core.MEM_CKPT: 3}[P.memory]
-        if call.memory == _core.MEM_CKPT and getattr(call, "recursive", False):
-            entry = 4
+        entry = {_core.MEM_FULL: 1, _core.MEM_BOUNDARY: 2, _core.MEM_CKPT: 3, _core.MEM_BINOMIAL: 4}[P.memory]
         name = call.names[entry]
         if call.illumination:
             P.flags |= 1   # before the query: the workspace depends on it
@@ -394,6 +392,11 @@ class PropTorch(torch.nn.Module):
             P.memory = _core.MEM_FULL
         elif isinstance(mem, BoundarySaving):
             P.memory = _core.MEM_BOUNDARY
+        elif mem.mode == "recursive":
+            # a fixed budget of `count` snapshot slots, reversed on a binomial
+            # schedule (the reference bisects recursively, eq_driver.cuh:1055-1259)
+            P.memory = _core.MEM_BINOMIAL
+            P.ckpt_interval = max(1, min(int(mem.count), max(nt, 1)))
         else:
             P.memory = _core.MEM_CKPT
             P.ckpt_interval = mem.interval(nt)
@@ -410,7 +413,6 @@ class PropTorch(torch.nn.Module):
                      nrec=nrec, nfield=len(self.receiver_type) if isinstance(eq, Elastic) else 1,
                      device=dev, memory=int(P.memory), requires_backward=bool(requires),
                      illumination=self.compute_illumination)
-        call.recursive = isinstance(mem, Ckpt) and mem.mode == "recursive"
         self._last_call = call
         out = _Wrapper.apply(call, w, *padded)
         return out

@@ -30,6 +30,8 @@ def _dev():
 
 def _memory(name):
     from sweep_amd.options import BoundarySaving, Ckpt, Full
+    if name.startswith("binomial"):   # "binomial<k>": Ckpt(mode='recursive', count=k)
+        return Ckpt(mode="recursive", count=int(name[len("binomial"):]))
     return {"full": Full(), "bs": BoundarySaving(), "ckpt": Ckpt(chunks=7)}[name]
 
 
@@ -133,6 +135,19 @@ def test_memory_modes_agree():
     assert np.array_equal(full["grad_wavelet"], bs["grad_wavelet"])
     assert np.array_equal(full["grad_vp"], ck["grad_vp"])          # the replay is the forward, bit for bit
     assert rel_l2(bs["grad_vp"], full["grad_vp"]) <= 1e-4          # the reconstruction is not
+
+
+@pytest.mark.parametrize("slots", [1, 3, 16, 400])
+def test_binomial_checkpointing_equals_full(slots):
+    """Ckpt(mode='recursive', count=k) replays the forward from k snapshots on
+    the binomial schedule: the gradients are the full-storage ones bit for bit."""
+    dev = _dev()
+    print(f"[acoustic2d] binomial slots={slots}", flush=True)
+    z = np.load(GOLDEN / "acoustic2d_het_order8.npz")
+    full = _run(z, "full", dev)
+    bn = _run(z, f"binomial{slots}", dev)
+    for k in full:
+        assert np.array_equal(full[k], bn[k]), k
 
 
 def test_run_to_run_bit_exact():

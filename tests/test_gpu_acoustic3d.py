@@ -28,6 +28,8 @@ def _dev():
 
 def _memory(name):
     from sweep_amd.options import BoundarySaving, Ckpt, Full
+    if name.startswith("binomial"):   # "binomial<k>": Ckpt(mode='recursive', count=k)
+        return Ckpt(mode="recursive", count=int(name[len("binomial"):]))
     return {"full": Full(), "bs": BoundarySaving(), "ckpt": Ckpt(chunks=5)}[name]
 
 
@@ -158,9 +160,12 @@ def test_memory_modes_agree():
     full = _gpu_run(*args, "full", *io)
     bs = _gpu_run(*args, "bs", *io)
     ck = _gpu_run(*args, "ckpt", *io)
+    bn = _gpu_run(*args, "binomial4", *io)
     assert np.array_equal(full["rec"], bs["rec"]) and np.array_equal(full["rec"], ck["rec"])
     assert np.array_equal(full["grad_wavelet"], bs["grad_wavelet"])
     assert np.array_equal(full["grad_vp"], ck["grad_vp"])
+    for k in full:   # binomial checkpointing replays the forward bit for bit
+        assert np.array_equal(full[k], bn[k]), k
     assert rel_l2(bs["grad_vp"], full["grad_vp"]) <= 1e-4
 
 

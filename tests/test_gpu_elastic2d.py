@@ -24,6 +24,8 @@ def _dev():
 
 def _memory(name):
     from sweep_amd.options import Ckpt, Full
+    if name.startswith("binomial"):   # "binomial<k>": Ckpt(mode='recursive', count=k)
+        return Ckpt(mode="recursive", count=int(name[len("binomial"):]))
     return {"full": Full(), "ckpt": Ckpt(chunks=7)}[name]
 
 
@@ -126,6 +128,10 @@ def test_ckpt_equals_full():
     ck = _gpu_run(*args, "ckpt", *io)
     for k in full:
         assert np.array_equal(full[k], ck[k]), k
+    for slots in (1, 5, 240):   # binomial checkpointing replays the forward bit for bit
+        bn = _gpu_run(*args, f"binomial{slots}", *io)
+        for k in full:
+            assert np.array_equal(full[k], bn[k]), (slots, k)
 
 
 def test_refusals():
