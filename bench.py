@@ -61,7 +61,26 @@ KERNEL_NAMES_E2D = {0: "fwd (velocity + stress step)", 1: "adj (stress/velocity 
 
 
 # ----------------------------------------------------------------- problems
-def make_problem(W, dev, nt):
+def memory_strategy(spec, elastic):
+    """'bs' | 'ckpt:<chunks>' | 'binomial:<slots>' | 'full' -> an options object
+    (None: the workload's default: checkpointing with 100 chunks for the
+    elastic workload, boundary saving for the acoustic ones)."""
+    from sweep_amd.options import BoundarySaving, Ckpt, Full
+    if spec is None:
+        return Ckpt(chunks=100) if elastic else BoundarySaving()
+    kind, _, arg = spec.partition(":")
+    if kind == "bs":
+        return BoundarySaving()
+    if kind == "full":
+        return Full()
+    if kind == "ckpt":
+        return Ckpt(chunks=int(arg or 100))
+    if kind == "binomial":
+        return Ckpt(mode="recursive", count=int(arg or 100))
+    raise SystemExit(f"unknown --memory {spec!r}")
+
+
+def make_problem(W, dev, nt, memory=None):
     """(solver, vp_true, vp_init, sources, receivers, mask or None, wavelet)."""
     from sweep_amd import synthetic
     from sweep_amd.equations import Acoustic, Acoustic3D
@@ -79,7 +98,7 @@ def make_problem(W, dev, nt):
         src, rec, mask = synthetic.surface_acquisition(W["shape"], W["nshots"], seed=W["seed"],
                                                        nrec_range=(W["nrec"], W["nrec"]))
         solver = PropTorch(Elastic(spatial_order=W["order"], device=dev), shape=W["shape"], dh=W["dh"], dt=W["dt"],
-                           device=dev, abcn=W["abcn"], memory=Ckpt(chunks=100))
+                           device=dev, abcn=W["abcn"], memory=memory_strategy(memory, True))
         true = [vp_true, vs_true, rho_true]
         init = [synthetic.smoothed(m) for m in true]
         return solver, true, init, src, rec, None, wav
@@ -94,7 +113,7 @@ def make_problem(W, dev, nt):
         eq = Acoustic3D(spatial_order=W["order"], device=dev)
     vp_init = synthetic.smoothed(vp_true)
     solver = PropTorch(eq, shape=W["shape"], dh=W["dh"], dt=W["dt"], device=dev, abcn=W["abcn"],
-                       memory=BoundarySaving())
+                       memory=memory_strategy(memory, False))
     return solver, [vp_true], [vp_init], src, rec, mask, wav
 
 
@@ -235,6 +254,9 @@ def main():
     ap.add_argument("--cpu-sample-nt", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None, help="override nt (profiling runs only)")
     ap.add_argument("--shots-per-launch", type=int, default=None)
+    ap.add_argument("--memory", default=None,
+                    help="gradient memory strategy: bs | full | ckpt:<chunks> | binomial:<slots> "
+                         "(default: the workload's)")
     ap.add_argument("--pmc-launch", action="store_true",
                     help="counter runs: a 1 GiB device copy (the calibration of FETCH_SIZE / WRITE_SIZE), "
                          "then one untimed gradient launch; prints no JSON")
@@ -249,7 +271,7 @@ def main():
     nt = args.nt or W["nt"]
     per_launch = args.shots_per_launch or W["per_launch"]
     dev = torch.device("cuda:0")
-    solver, vp_true, vp_init, src, rec, mask, wav = make_problem(W, dev, nt)
+    solver, vp_true, vp_init, src, rec, mask, wav = make_problem(W, dev, nt, args.memory)
     nshots = src.shape[0]
     if args.pmc_launch:
         ca = torch.ones(1 << 28, device=dev)
@@ -302,6 +324,7 @@ def main():
 
     P = solver._last_call.problem
     grid = (P.nz, P.nx) if P.ndim == 2 else (P.nz, P.ny, P.nx)
+    tape_bytes = int(P.tape_bytes)
     elastic = W["name"].startswith("elastic")
     names = KERNEL_NAMES_E2D if elastic else KERNEL_NAMES
     abytes = elastic_bytes(P.B, grid, P.M, W["abcn"]) if elastic else algorithmic_bytes(P.B, grid, P.M, W["abcn"],
@@ -323,7 +346,8 @@ def main():
         "config": {"workload": W["name"] if args.nt is None else f"{W['name']}_nt{nt}",
                    "grid": list(W["shape"]), "runtime_grid": list(grid), "nt": nt, "shots": nshots,
                    "shots_per_launch": per_launch, "order": W["order"],
-                   "memory": "checkpoint" if elastic else "boundary_saving"},
+                   "memory": args.memory or ("ckpt:100" if elastic else "bs"),
+                   "tape_bytes_per_launch": tape_bytes},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload, dom, per_launch),
