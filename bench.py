@@ -63,7 +63,7 @@ KERNEL_NAMES_E2D = {0: "fwd (velocity + stress step)", 1: "adj (stress/velocity 
 
 # ----------------------------------------------------------------- problems
 def memory_strategy(spec, elastic):
-    """'bs' | 'ckpt:<chunks>' | 'binomial:<slots>' | 'full' -> an options object
+    """'bs' | 'bs_bf16' | 'ckpt:<chunks>' | 'binomial:<slots>' | 'full' -> an options object
     (None: the workload's default: checkpointing with 100 chunks for the
     elastic workload, boundary saving for the acoustic ones)."""
     from sweep_amd.options import BoundarySaving, Ckpt, Full
@@ -72,6 +72,8 @@ def memory_strategy(spec, elastic):
     kind, _, arg = spec.partition(":")
     if kind == "bs":
         return BoundarySaving()
+    if kind == "bs_bf16":
+        return BoundarySaving(storage_dtype="bf16")
     if kind == "full":
         return Full()
     if kind == "ckpt":
@@ -255,7 +257,7 @@ def main():
     ap.add_argument("--nt", type=int, default=None, help="override nt (profiling runs only)")
     ap.add_argument("--shots-per-launch", type=int, default=None)
     ap.add_argument("--memory", default=None,
-                    help="gradient memory strategy: bs | full | ckpt:<chunks> | binomial:<slots> "
+                    help="gradient memory strategy: bs | bs_bf16 | full | ckpt:<chunks> | binomial:<slots> "
                          "(default: the workload's)")
     ap.add_argument("--pmc-launch", action="store_true",
                     help="counter runs: a 1 GiB device copy (the calibration of FETCH_SIZE / WRITE_SIZE), "

@@ -41,6 +41,14 @@ enum {
                               (served by the *_backward_recursive_ckpt entries) */
 };
 
+/* swp_problem.flags */
+enum {
+    SWP_FLAG_ILLUMINATION = 1,   /* accumulate illumination in the backward */
+    SWP_FLAG_BF16_STRIPS = 2,    /* SWP_MEM_BOUNDARY strips in bfloat16, round to nearest even
+                                    (reference: BoundarySaving(storage_dtype='bf16'),
+                                    propagator/options.py:204-212, boundarysaver.cu:29) */
+};
+
 /* One propagation problem on the runtime grid.  Arrays are row-major with x
  * fastest; a per-shot array is (B, nz[, ny], nx).  Coordinates are runtime
  * cell indices in (x[, y], z) order, like the reference's sources_loc /
@@ -70,7 +78,7 @@ typedef struct swp_problem {
     const int32_t *receivers_loc;   /* (B, nrec, ndim) */
     int32_t memory;          /* SWP_MEM_* */
     int32_t ckpt_interval;   /* SWP_MEM_CKPT: steps between snapshots */
-    int32_t flags;           /* bit 0: accumulate illumination in the backward */
+    int32_t flags;           /* SWP_FLAG_ILLUMINATION, SWP_FLAG_BF16_STRIPS */
     int32_t reserved;
     /* Wavefield indices the sources inject into and the receivers record
      * (reference: source_field_indices / receiver_field_indices,

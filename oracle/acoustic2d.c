@@ -37,6 +37,7 @@
  * Full == BS agreement.
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -117,6 +118,20 @@ static inline int phys_z1(const orc_a2d *g) { return g->nz - g->pad[1] - g->M; }
  * one check. */
 static int g_edge_mode = 0;
 void orc_set_edge_mode(int mode) { g_edge_mode = mode; }
+
+int orc_g_strip_bf16 = 0;
+void orc_set_strip_bf16(int on) { orc_g_strip_bf16 = on; }
+
+float orc_bf16_round(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) u = (u | 0x00400000u) & 0xffff0000u;   /* quiet NaN */
+    else u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    float y;
+    memcpy(&y, &u, 4);
+    return y;
+}
 
 static inline float tap0(const float *u, long i, int ok) { return ok ? u[i] : 0.f; }
 
@@ -342,13 +357,15 @@ static void strips_io(const orc_a2d *g, float *u, float *slot, int restore)
         for (int c = 0; c < nxp; ++c) {
             float *t = &top[(long)r * nxp + c], *b = &bot[(long)r * nxp + c];
             float *ut = &u[IDX(z0 + r, x0 + c)], *ub = &u[IDX(z1 - W + r, x0 + c)];
-            if (restore) { *ut = *t; *ub = *b; } else { *t = *ut; *b = *ub; }
+            if (restore) { *ut = *t; *ub = *b; }
+            else { *t = orc_g_strip_bf16 ? orc_bf16_round(*ut) : *ut; *b = orc_g_strip_bf16 ? orc_bf16_round(*ub) : *ub; }
         }
     for (int r = 0; r < nzp; ++r)
         for (int c = 0; c < W; ++c) {
             float *l = &lef[(long)r * W + c], *rr = &rig[(long)r * W + c];
             float *ul = &u[IDX(z0 + r, x0 + c)], *ur = &u[IDX(z0 + r, x1 - W + c)];
-            if (restore) { *ul = *l; *ur = *rr; } else { *l = *ul; *rr = *ur; }
+            if (restore) { *ul = *l; *ur = *rr; }
+            else { *l = orc_g_strip_bf16 ? orc_bf16_round(*ul) : *ul; *rr = orc_g_strip_bf16 ? orc_bf16_round(*ur) : *ur; }
         }
 }
 

@@ -16,6 +16,8 @@
 
 #include "oracle.h"
 
+extern int orc_g_strip_bf16;   /* acoustic2d.c, set by orc_set_strip_bf16 */
+
 #define I3(iz, iy, ix) (((long)(iz) * ny + (iy)) * nx + (ix))
 
 static inline float lapf(const orc_a3d *g, const float *u, long i, long s, float h)
@@ -224,7 +226,8 @@ static void strips3(const orc_a3d *g, float *u, float *slot, int restore)
                     else in = hi ? ix >= x1 - W : ix < x0 + W;
                     if (!in) continue;
                     float *cell = &u[I3(iz, iy, ix)];
-                    if (restore) *cell = p[cnt]; else p[cnt] = *cell;
+                    if (restore) *cell = p[cnt];
+                    else p[cnt] = orc_g_strip_bf16 ? orc_bf16_round(*cell) : *cell;
                     ++cnt;
                 }
         p += cnt;

@@ -38,6 +38,14 @@ long orc_a2d_strip_cells(const orc_a2d *g);
 /* Test hook: 1 = eager-path edge treatment in the forward (see acoustic2d.c). */
 void orc_set_edge_mode(int mode);
 
+/* 1: boundary-saving strips are stored as bfloat16, rounded to nearest with
+ * ties to even and a NaN kept a quiet NaN -- the reference's
+ * __float2bfloat16 (boundarysaver.cu:29; its host stand-in
+ * csrc/host/include/cuda_bf16.h) -- both for the 2-D and the 3-D oracle. */
+void orc_set_strip_bf16(int on);
+/* x rounded to bfloat16 and widened back to float. */
+float orc_bf16_round(float x);
+
 void orc_a2d_fwd_step(const orc_a2d *g, const float *vp, const float *u_prev, const float *u_now,
                       float *u_next, const float *psix, const float *psiz, float *psixn,
                       float *psizn, float *zetax, float *zetaz, float *u_this);

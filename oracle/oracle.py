@@ -63,6 +63,10 @@ def lib(variant="ieee"):
         L.orc_a2d_fwd_step.restype = None
         L.orc_a2d_adj_step.restype = None
         L.orc_set_edge_mode.restype = None
+        L.orc_set_strip_bf16.restype = None
+        L.orc_set_strip_bf16.argtypes = [ctypes.c_int]
+        L.orc_bf16_round.restype = ctypes.c_float
+        L.orc_bf16_round.argtypes = [ctypes.c_float]
         L.orc_a3d_forward.restype = None
         L.orc_a3d_gradient.restype = ctypes.c_int
         L.orc_e2d_forward.restype = None

@@ -21,6 +21,7 @@ _F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.POINTER(ctypes.c_int32)
 
 MEM_NONE, MEM_FULL, MEM_BOUNDARY, MEM_CKPT, MEM_BINOMIAL = 0, 1, 2, 3, 4
+FLAG_ILLUMINATION, FLAG_BF16_STRIPS = 1, 2
 
 
 class SwpProblem(ctypes.Structure):

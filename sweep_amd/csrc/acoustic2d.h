@@ -21,6 +21,7 @@ struct Geo {
     int64_t nm;                // cells per shot in the caller's arrays (nz * nx)
     int64_t mstride;           // model stride per shot in the core's copy (0: shared model)
     int64_t ns;                // strip cells per shot per step
+    int sbf16;                 // strips stored as bfloat16 (SWP_FLAG_BF16_STRIPS)
     float dt, dx, dz;
     float inv_dx2, inv_dz2, inv_dx, inv_dz, dt2, inv_dt2;
 };

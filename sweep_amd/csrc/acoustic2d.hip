@@ -47,6 +47,7 @@ Geo make_geo(const swp_problem &p)
     g.nxp = g.X1 - g.X0;
     g.nzp = g.Z1 - g.Z0;
     g.ns = 2LL * g.W * g.nxp + 2LL * g.W * g.nzp;
+    g.sbf16 = (p.flags & SWP_FLAG_BF16_STRIPS) != 0;
     g.inv_dx2 = 1.0f / (g.dx * g.dx);
     g.inv_dz2 = 1.0f / (g.dz * g.dz);
     g.inv_dx = 1.0f / g.dx;
@@ -178,18 +179,18 @@ __global__ void __launch_bounds__(256) fwd_kernel(Geo g, Coef c, Prof pr, StepFi
     const int64_t nx = g.px;   // row pitch
 
     if (strip) {   // boundary save of u_now (driver_traits.cuh:248-265, strip.cuh)
-        float *s = strip + int64_t(b) * g.ns;
+        float *s = strip_at(strip, int64_t(b) * g.ns, g.sbf16);
         const float v = un[i];
         const bool inx = ix >= g.X0 && ix < g.X1, inz = iz >= g.Z0 && iz < g.Z1;
         if (inx && iz >= g.Z0 && iz < g.Z0 + g.W)
-            s[int64_t(iz - g.Z0) * g.nxp + (ix - g.X0)] = v;
+            strip_put(s, int64_t(iz - g.Z0) * g.nxp + (ix - g.X0), v, g.sbf16);
         if (inx && iz >= g.Z1 - g.W && iz < g.Z1)
-            s[int64_t(g.W) * g.nxp + int64_t(iz - (g.Z1 - g.W)) * g.nxp + (ix - g.X0)] = v;
+            strip_put(s, int64_t(g.W) * g.nxp + int64_t(iz - (g.Z1 - g.W)) * g.nxp + (ix - g.X0), v, g.sbf16);
         if (inz && ix >= g.X0 && ix < g.X0 + g.W)
-            s[2LL * g.W * g.nxp + int64_t(iz - g.Z0) * g.W + (ix - g.X0)] = v;
+            strip_put(s, 2LL * g.W * g.nxp + int64_t(iz - g.Z0) * g.W + (ix - g.X0), v, g.sbf16);
         if (inz && ix >= g.X1 - g.W && ix < g.X1)
-            s[2LL * g.W * g.nxp + int64_t(g.nzp) * g.W + int64_t(iz - g.Z0) * g.W +
-              (ix - (g.X1 - g.W))] = v;
+            strip_put(s, 2LL * g.W * g.nxp + int64_t(g.nzp) * g.W + int64_t(iz - g.Z0) * g.W + (ix - (g.X1 - g.W)),
+                      v, g.sbf16);
     }
 
     const float lap_x = lap_fwd<MS>(c, un, i, 1, g.inv_dx2, M);
@@ -382,12 +383,13 @@ __global__ void __launch_bounds__(256) strip_restore_kernel(
     else if ((t -= n_left) < n_right) { ix = (g.X1 - wxh) + t % wxh; iz = zi0 + t / wxh; }
     else return;
     // saved value: from the top/bottom band when the cell is in one, else a side band
-    const float *s = slot + int64_t(b) * g.ns;
-    float val;
-    if (iz < g.Z0 + g.W) val = s[int64_t(iz - g.Z0) * g.nxp + (ix - g.X0)];
-    else if (iz >= g.Z1 - g.W) val = s[int64_t(g.W) * g.nxp + int64_t(iz - (g.Z1 - g.W)) * g.nxp + (ix - g.X0)];
-    else if (ix < g.X0 + g.W) val = s[2LL * g.W * g.nxp + int64_t(iz - g.Z0) * g.W + (ix - g.X0)];
-    else val = s[2LL * g.W * g.nxp + int64_t(g.nzp) * g.W + int64_t(iz - g.Z0) * g.W + (ix - (g.X1 - g.W))];
+    const float *s = strip_at(slot, int64_t(b) * g.ns, g.sbf16);
+    int64_t k;
+    if (iz < g.Z0 + g.W) k = int64_t(iz - g.Z0) * g.nxp + (ix - g.X0);
+    else if (iz >= g.Z1 - g.W) k = int64_t(g.W) * g.nxp + int64_t(iz - (g.Z1 - g.W)) * g.nxp + (ix - g.X0);
+    else if (ix < g.X0 + g.W) k = 2LL * g.W * g.nxp + int64_t(iz - g.Z0) * g.W + (ix - g.X0);
+    else k = 2LL * g.W * g.nxp + int64_t(g.nzp) * g.W + int64_t(iz - g.Z0) * g.W + (ix - (g.X1 - g.W));
+    float val = strip_get(s, k, g.sbf16);
     for (int k = 0; k < nsrc; ++k) {
         const int32_t *l = src_loc + (int64_t(b) * nsrc + k) * 2;
         if (l[0] == ix && l[1] == iz) val += -src[(int64_t(b) * nsrc + k) * g.nt + it];
@@ -572,7 +574,8 @@ static Tape carve_tape(Carver &cv, const swp_problem &p, const Geo &g)
     if (p.memory == SWP_MEM_FULL) {
         t.u_this = cv.take<float>(int64_t(g.nt) * N);
     } else if (p.memory == SWP_MEM_BOUNDARY) {
-        t.strips = cv.take<float>(int64_t(g.nt) * g.B * g.ns);
+        const int64_t cells = int64_t(g.nt) * g.B * g.ns;
+        t.strips = g.sbf16 ? reinterpret_cast<float *>(cv.take<uint16_t>(cells)) : cv.take<float>(cells);
         t.last_two = cv.take<float>(2 * N);
     } else if (p.memory == SWP_MEM_CKPT) {
         SWP_CHECK(p.ckpt_interval >= 1, "ckpt_interval must be >= 1");
@@ -717,7 +720,7 @@ void forward(const Ctx &x, const swp_problem &p_in, float *record)
             snapshot(g, s, t.snaps + int64_t(it / p.ckpt_interval) * 6 * N);
         for (; sp < spine.size() && spine[sp].step == it; ++sp) snapshot(g, s, t.snaps + int64_t(spine[sp].slot) * 6 * N);
         forward_step(x, p, s, it, t.u_this ? t.u_this + int64_t(it) * N : nullptr,
-                     t.strips ? t.strips + int64_t(it) * g.B * g.ns : nullptr, record);
+                     t.strips ? strip_at(t.strips, int64_t(it) * g.B * g.ns, g.sbf16) : nullptr, record);
     }
     if (t.last_two) {
         copy(t.last_two, s.u[0], N);
@@ -904,7 +907,7 @@ static void backward_run(const Ctx &x, const swp_problem &p, const swp_backward_
         if (n_strip > 0) {
             TimedScope ts(K_OTHER);
             hipLaunchKernelGGL(strip_restore_kernel, dim3(unsigned((n_strip + 255) / 256), unsigned(g.B)),
-                               dim3(256), 0, stream(), g, t.strips + int64_t(it - 1) * g.B * g.ns, r[2],
+                               dim3(256), 0, stream(), g, strip_at(t.strips, int64_t(it - 1) * g.B * g.ns, g.sbf16), r[2],
                                r[1], r[0], lam, p.models[0], grad, p.source, p.sources_loc, p.nsrc, it,
                                wzl, wzh, wxl, wxh);
             SWP_LAUNCHED();

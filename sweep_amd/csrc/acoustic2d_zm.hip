@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
         q[k] = un[int64_t(zr) * px + xc];
         pq[k] = seg_z ? pr.az[zr] * psiz[int64_t(zr) * px + xc] : 0.f;
     }
-    float *s_out = STRIP ? strip_out + int64_t(b) * g.ns : nullptr;
+    float *s_out = STRIP ? strip_at(strip_out, int64_t(b) * g.ns, g.sbf16) : nullptr;
 
     float n_q, n_pq = 0.f, n_h, n_hp = 0.f, n_px = 0.f, n_up, n_v, n_zx = 0.f, n_zz = 0.f, n_pz = 0.f;
     auto issue = [&](int zr) {
@@ -217,12 +217,15 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
         if (STRIP && active) {   // boundary save of u_now (strip.cuh)
             const float u0 = q[M];
             const bool inx = x >= g.X0 && x < g.X1, inz = z >= g.Z0 && z < g.Z1;
-            if (inx && z < g.Z0 + g.W && z >= g.Z0) s_out[int64_t(z - g.Z0) * g.nxp + (x - g.X0)] = u0;
+            const bool bf = g.sbf16;
+            if (inx && z < g.Z0 + g.W && z >= g.Z0) strip_put(s_out, int64_t(z - g.Z0) * g.nxp + (x - g.X0), u0, bf);
             if (inx && z >= g.Z1 - g.W && z < g.Z1)
-                s_out[int64_t(g.W) * g.nxp + int64_t(z - (g.Z1 - g.W)) * g.nxp + (x - g.X0)] = u0;
-            if (inz && x >= g.X0 && x < g.X0 + g.W) s_out[2LL * g.W * g.nxp + int64_t(z - g.Z0) * g.W + (x - g.X0)] = u0;
+                strip_put(s_out, int64_t(g.W) * g.nxp + int64_t(z - (g.Z1 - g.W)) * g.nxp + (x - g.X0), u0, bf);
+            if (inz && x >= g.X0 && x < g.X0 + g.W)
+                strip_put(s_out, 2LL * g.W * g.nxp + int64_t(z - g.Z0) * g.W + (x - g.X0), u0, bf);
             if (inz && x >= g.X1 - g.W && x < g.X1)
-                s_out[2LL * g.W * g.nxp + int64_t(g.nzp) * g.W + int64_t(z - g.Z0) * g.W + (x - (g.X1 - g.W))] = u0;
+                strip_put(s_out, 2LL * g.W * g.nxp + int64_t(g.nzp) * g.W + int64_t(z - g.Z0) * g.W + (x - (g.X1 - g.W)),
+                          u0, bf);
         }
         const float lap_x = lap_taps<M>(c, [&](int k) { return row[M + lane + k]; }, g.inv_dx2);
         const float lap_z = lap_taps<M>(c, [&](int k) { return q[M + k]; }, g.inv_dz2);

@@ -22,6 +22,7 @@ struct Geo {
     int64_t nm;                         // cells per shot in the caller's arrays (nz * ny * nx)
     int64_t mstride;                    // model stride per shot (0: shared model)
     int64_t ns;                         // strip cells per shot per step (six faces)
+    int sbf16;                          // strips stored as bfloat16 (SWP_FLAG_BF16_STRIPS)
     float dt, dx, dy, dz;
     float inv_dx2, inv_dy2, inv_dz2, inv_dx, inv_dy, inv_dz, dt2, inv_dt2;
 };
@@ -71,18 +72,26 @@ struct Box {
 // Strip layout of one shot and step (ns cells): faces z-lo, z-hi as
 // (W, nyp, nxp), y-lo, y-hi as (nzp, W, nxp), x-lo, x-hi as (nzp, nyp, W);
 // a cell on an edge or corner is stored in each face that holds it.
-__device__ __forceinline__ void save_strips(const Geo &g, float *__restrict__ s, int ix, int iy, int iz, float v)
+template <bool bf>
+__device__ __forceinline__ void save_strips_as(const Geo &g, float *__restrict__ s, int ix, int iy, int iz, float v)
 {
-    if (ix < g.X0 || ix >= g.X1 || iy < g.Y0 || iy >= g.Y1 || iz < g.Z0 || iz >= g.Z1) return;
     const int64_t W = g.W, nxp = g.nxp, nyp = g.nyp, nzp = g.nzp;
     const int64_t x = ix - g.X0, y = iy - g.Y0, z = iz - g.Z0;
     const int64_t fz = W * nyp * nxp, fy = nzp * W * nxp, fx = nzp * nyp * W;
-    if (z < W) s[(z * nyp + y) * nxp + x] = v;
-    if (iz >= g.Z1 - g.W) s[fz + ((iz - (g.Z1 - g.W)) * nyp + y) * nxp + x] = v;
-    if (y < W) s[2 * fz + (z * W + y) * nxp + x] = v;
-    if (iy >= g.Y1 - g.W) s[2 * fz + fy + (z * W + (iy - (g.Y1 - g.W))) * nxp + x] = v;
-    if (x < W) s[2 * fz + 2 * fy + (z * nyp + y) * W + x] = v;
-    if (ix >= g.X1 - g.W) s[2 * fz + 2 * fy + fx + (z * nyp + y) * W + (ix - (g.X1 - g.W))] = v;
+    if (z < W) strip_put(s, (z * nyp + y) * nxp + x, v, bf);
+    if (iz >= g.Z1 - g.W) strip_put(s, fz + ((iz - (g.Z1 - g.W)) * nyp + y) * nxp + x, v, bf);
+    if (y < W) strip_put(s, 2 * fz + (z * W + y) * nxp + x, v, bf);
+    if (iy >= g.Y1 - g.W) strip_put(s, 2 * fz + fy + (z * W + (iy - (g.Y1 - g.W))) * nxp + x, v, bf);
+    if (x < W) strip_put(s, 2 * fz + 2 * fy + (z * nyp + y) * W + x, v, bf);
+    if (ix >= g.X1 - g.W) strip_put(s, 2 * fz + 2 * fy + fx + (z * nyp + y) * W + (ix - (g.X1 - g.W)), v, bf);
+}
+
+// The stencil kernels save float32 (with bf16 strips the driver packs each
+// step's strips afterwards, pack_bf16).
+__device__ __forceinline__ void save_strips(const Geo &g, float *__restrict__ s, int ix, int iy, int iz, float v)
+{
+    if (ix < g.X0 || ix >= g.X1 || iy < g.Y0 || iy >= g.Y1 || iz < g.Z0 || iz >= g.Z1) return;
+    save_strips_as<false>(g, s, ix, iy, iz, v);
 }
 
 __device__ __forceinline__ float load_strip(const Geo &g, const float *__restrict__ s, int ix, int iy, int iz)
@@ -90,12 +99,13 @@ __device__ __forceinline__ float load_strip(const Geo &g, const float *__restric
     const int64_t W = g.W, nxp = g.nxp, nyp = g.nyp, nzp = g.nzp;
     const int64_t x = ix - g.X0, y = iy - g.Y0, z = iz - g.Z0;
     const int64_t fz = W * nyp * nxp, fy = nzp * W * nxp, fx = nzp * nyp * W;
-    if (z < W) return s[(z * nyp + y) * nxp + x];
-    if (iz >= g.Z1 - g.W) return s[fz + ((iz - (g.Z1 - g.W)) * nyp + y) * nxp + x];
-    if (y < W) return s[2 * fz + (z * W + y) * nxp + x];
-    if (iy >= g.Y1 - g.W) return s[2 * fz + fy + (z * W + (iy - (g.Y1 - g.W))) * nxp + x];
-    if (x < W) return s[2 * fz + 2 * fy + (z * nyp + y) * W + x];
-    return s[2 * fz + 2 * fy + fx + (z * nyp + y) * W + (ix - (g.X1 - g.W))];
+    const bool bf = g.sbf16;
+    if (z < W) return strip_get(s, (z * nyp + y) * nxp + x, bf);
+    if (iz >= g.Z1 - g.W) return strip_get(s, fz + ((iz - (g.Z1 - g.W)) * nyp + y) * nxp + x, bf);
+    if (y < W) return strip_get(s, 2 * fz + (z * W + y) * nxp + x, bf);
+    if (iy >= g.Y1 - g.W) return strip_get(s, 2 * fz + fy + (z * W + (iy - (g.Y1 - g.W))) * nxp + x, bf);
+    if (x < W) return strip_get(s, 2 * fz + 2 * fy + (z * nyp + y) * W + x, bf);
+    return strip_get(s, 2 * fz + 2 * fy + fx + (z * nyp + y) * W + (ix - (g.X1 - g.W)), bf);
 }
 
 bool available();

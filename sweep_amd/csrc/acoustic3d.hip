@@ -59,6 +59,7 @@ Geo make_geo(const swp_problem &p)
     g.mstride = p.model_batched ? g.n : 0;
     const int64_t W = g.W;
     g.ns = 2 * W * g.nyp * g.nxp + 2 * W * g.nzp * g.nxp + 2 * W * g.nzp * g.nyp;
+    g.sbf16 = (p.flags & SWP_FLAG_BF16_STRIPS) != 0;
     g.dt = p.dt;
     g.dx = p.dx;
     g.dy = p.dy;
@@ -319,7 +320,7 @@ __global__ void __launch_bounds__(256) strip_restore_kernel(
     } else {
         return;
     }
-    float val = load_strip(g, slot + int64_t(b) * g.ns, ix, iy, iz);
+    float val = load_strip(g, strip_at(slot, int64_t(b) * g.ns, g.sbf16), ix, iy, iz);
     for (int k = 0; k < nsrc; ++k) {
         const int32_t *l = src_loc + (int64_t(b) * nsrc + k) * 3;
         if (l[0] == ix && l[1] == iy && l[2] == iz) val += -src[(int64_t(b) * nsrc + k) * g.nt + it];
@@ -530,7 +531,8 @@ static Tape carve_tape(Carver &cv, const swp_problem &p, const Geo &g)
     if (p.memory == SWP_MEM_FULL) {
         t.u_this = cv.take<float>(int64_t(g.nt) * N);
     } else if (p.memory == SWP_MEM_BOUNDARY) {
-        t.strips = cv.take<float>(int64_t(g.nt) * g.B * g.ns);
+        const int64_t cells = int64_t(g.nt) * g.B * g.ns;
+        t.strips = g.sbf16 ? reinterpret_cast<float *>(cv.take<uint16_t>(cells)) : cv.take<float>(cells);
         t.last_two = cv.take<float>(2 * N);
     } else if (p.memory == SWP_MEM_CKPT) {
         SWP_CHECK(p.ckpt_interval >= 1, "ckpt_interval must be >= 1");
@@ -549,12 +551,14 @@ static Tape carve_tape(Carver &cv, const swp_problem &p, const Geo &g)
 struct IoBufs {
     float *vp = nullptr;                                 // (B or 1, n)
     float *grad = nullptr, *S = nullptr, *R = nullptr;   // (B, n), backward only (S, R with illumination)
+    float *stage = nullptr;                              // (B, ns) float32 strips of one step, bf16 tape only
 };
 
 static IoBufs carve_io(Carver &cv, const swp_problem &p, const Geo &g, int kind)
 {
     IoBufs b;
     b.vp = cv.take<float>((p.model_batched ? int64_t(g.B) : 1) * g.n);
+    if (kind == 0 && p.memory == SWP_MEM_BOUNDARY && g.sbf16) b.stage = cv.take<float>(int64_t(g.B) * g.ns);
     if (kind == 1) {
         const int64_t N = int64_t(g.B) * g.n;
         b.grad = cv.take<float>(N);
@@ -685,7 +689,8 @@ void forward(const swp_problem &p_in, float *record)
     FwdState s;
     carve_fwd(ws, g, s);
     zero_fwd(g, s);
-    const swp_problem p = with_pitched_model(p_in, g, carve_io(ws, p_in, g, 0));
+    const IoBufs io = carve_io(ws, p_in, g, 0);
+    const swp_problem p = with_pitched_model(p_in, g, io);
     Carver tc(p.tape, p.tape_bytes);
     Tape t = carve_tape(tc, p, g);
     const int64_t N = int64_t(g.B) * g.n;
@@ -697,8 +702,10 @@ void forward(const swp_problem &p_in, float *record)
             snapshot(g, s, t.snaps + int64_t(it / p.ckpt_interval) * kSnapFields * N);
         for (; sp < spine.size() && spine[sp].step == it; ++sp)
             snapshot(g, s, t.snaps + int64_t(spine[sp].slot) * kSnapFields * N);
+        float *tape_strips = t.strips ? strip_at(t.strips, int64_t(it) * g.B * g.ns, g.sbf16) : nullptr;
         forward_step(x, p, s, it, t.u_this ? t.u_this + int64_t(it) * N : nullptr,
-                     t.strips ? t.strips + int64_t(it) * g.B * g.ns : nullptr, record);
+                     io.stage ? io.stage : tape_strips, record);
+        if (io.stage) pack_bf16(tape_strips, io.stage, int64_t(g.B) * g.ns);
     }
     if (t.last_two) {
         copy(t.last_two, s.u[0], N);
@@ -887,7 +894,7 @@ static void backward_run(const Ctx &x, const swp_problem &p, const swp_backward_
         if (n_strip > 0) {
             TimedScope ts(K_OTHER);
             hipLaunchKernelGGL(strip_restore_kernel, dim3(unsigned((n_strip + 255) / 256), unsigned(g.B)), dim3(256),
-                               0, stream(), g, t.strips + int64_t(it - 1) * g.B * g.ns, r[2], r[1], r[0], lam,
+                               0, stream(), g, strip_at(t.strips, int64_t(it - 1) * g.B * g.ns, g.sbf16), r[2], r[1], r[0], lam,
                                p.models[0], grad, p.source, p.sources_loc, p.nsrc, it, sw);
             SWP_LAUNCHED();
         }

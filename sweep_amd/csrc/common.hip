@@ -110,6 +110,34 @@ void pitch_add(float *dst, int64_t dpitch, const float *src, int64_t spitch, int
     launch_pitch<true>(dst, dpitch, src, spitch, rows, width);
 }
 
+// ----------------------------------------------------------------- bf16 pack
+// Pairs (k, k + 1) from k = head go out as one 4-byte store (dst + head is
+// 4-byte aligned); a leading or trailing odd element goes alone.
+__global__ void __launch_bounds__(256) pack_bf16_kernel(uint16_t *__restrict__ dst, const float *__restrict__ src,
+                                                        int64_t n, int head)
+{
+    const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (t == 0 && head) dst[0] = bf16_rne(src[0]);
+    for (int64_t k = head + t * 2; k < n; k += int64_t(gridDim.x) * 512) {
+        if (k + 1 < n)
+            *reinterpret_cast<uint32_t *>(dst + k) = uint32_t(bf16_rne(src[k])) | (uint32_t(bf16_rne(src[k + 1])) << 16);
+        else
+            dst[k] = bf16_rne(src[k]);
+    }
+}
+
+void pack_bf16(float *dst, const float *src, int64_t n)
+{
+    if (n <= 0) return;
+    const uintptr_t d = reinterpret_cast<uintptr_t>(dst);
+    SWP_CHECK((d & 1) == 0 && (reinterpret_cast<uintptr_t>(src) & 3) == 0, "pack_bf16: unaligned buffers");
+    const int head = (d & 3) ? 1 : 0;
+    const int64_t blocks = std::min<int64_t>((n / 2 + 255) / 256 + 1, 8192);
+    hipLaunchKernelGGL(pack_bf16_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream(),
+                       reinterpret_cast<uint16_t *>(dst), src, n, head);
+    SWP_LAUNCHED();
+}
+
 // ----------------------------------------------------------------- timer
 KTimer &timer()
 {

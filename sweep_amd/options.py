@@ -13,8 +13,10 @@ names, fields and validation, for the parts this build implements.
   schedule: with r the smallest integer with C(count + r, count) >= nt, every
   step is re-run at most r times (the reference bisects recursively).
 
-Host/disk staging, lossy or compressed strip storage and tail truncation are
-storage tiers outside this build; asking for them raises.
+``BoundarySaving(storage_dtype='bf16')`` keeps the strips in bfloat16 (round
+to nearest even), half the bytes of fp32.  Host/disk staging, fp16/int8 or
+compressed strip storage and tail truncation are storage tiers outside this
+build; asking for them raises.
 """
 from __future__ import annotations
 
@@ -53,9 +55,9 @@ class BoundarySaving:
             raise NotImplementedError(
                 f"BoundarySaving(storage={self.storage!r}): host/disk staging is not part of this "
                 "build; the strips stay in HBM (288 GB per MI355X).")
-        if self.storage_dtype != "fp32":
+        if self.storage_dtype not in ("fp32", "bf16"):
             raise NotImplementedError(
-                f"BoundarySaving(storage_dtype={self.storage_dtype!r}) is not built; use fp32.")
+                f"BoundarySaving(storage_dtype={self.storage_dtype!r}) is not built; use fp32 or bf16.")
         if self.tail_steps or self.compress:
             raise NotImplementedError("BoundarySaving tail_steps / compress are not built.")
 

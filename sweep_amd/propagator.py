@@ -111,7 +111,7 @@ class _Wrapper(torch.autograd.Function):
         entry = {_core.MEM_FULL: 1, _core.MEM_BOUNDARY: 2, _core.MEM_CKPT: 3, _core.MEM_BINOMIAL: 4}[P.memory]
         name = call.names[entry]
         if call.illumination:
-            P.flags |= 1   # before the query: the workspace depends on it
+            P.flags |= _core.FLAG_ILLUMINATION   # before the query: the workspace depends on it
         ws, _ = _core.query(name, P)
         workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
         P.workspace, P.workspace_bytes = workspace.data_ptr(), ws
@@ -392,6 +392,8 @@ class PropTorch(torch.nn.Module):
             P.memory = _core.MEM_FULL
         elif isinstance(mem, BoundarySaving):
             P.memory = _core.MEM_BOUNDARY
+            if mem.storage_dtype == "bf16":
+                P.flags |= _core.FLAG_BF16_STRIPS
         elif mem.mode == "recursive":
             # a fixed budget of `count` snapshot slots, reversed on a binomial
             # schedule (the reference bisects recursively, eq_driver.cuh:1055-1259)

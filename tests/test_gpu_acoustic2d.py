@@ -19,7 +19,7 @@ from conftest import GOLDEN, cosine, rel_l2
 pytestmark = pytest.mark.gpu
 
 CASES = ["c1_order4", "het_order8", "het_order6", "het_order2", "het_order10"]
-MODES = {"full": 0, "bs": 1, "ckpt": 0}
+MODES = {"full": 0, "bs": 1, "ckpt": 0, "bs_bf16": 1}
 
 
 def _dev():
@@ -32,7 +32,8 @@ def _memory(name):
     from sweep_amd.options import BoundarySaving, Ckpt, Full
     if name.startswith("binomial"):   # "binomial<k>": Ckpt(mode='recursive', count=k)
         return Ckpt(mode="recursive", count=int(name[len("binomial"):]))
-    return {"full": Full(), "bs": BoundarySaving(), "ckpt": Ckpt(chunks=7)}[name]
+    return {"full": Full(), "bs": BoundarySaving(), "ckpt": Ckpt(chunks=7),
+            "bs_bf16": BoundarySaving(storage_dtype="bf16")}[name]
 
 
 def _prop(z, mode, dev, illumination=False):
@@ -69,13 +70,17 @@ def _oracle(name, mode):
     key = (name, mode)
     if key in _oracle_cache:
         return _oracle_cache[key]
-    from oracle.oracle import a2d_from_golden
+    from oracle.oracle import a2d_from_golden, lib
     z = np.load(GOLDEN / f"acoustic2d_{name}.npz")
     P, wav, sl, rl = a2d_from_golden(z)
     obs = P.forward(P.pad_model(z["vp_true"]), wav, sl, rl)
     vpi = P.pad_model(z["vp_init"])
     pred = P.forward(vpi, wav, sl, rl)
-    out = P.gradient(MODES[mode], vpi, wav, sl, rl, pred - obs, illumination=True)
+    lib().orc_set_strip_bf16(1 if mode == "bs_bf16" else 0)
+    try:
+        out = P.gradient(MODES[mode], vpi, wav, sl, rl, pred - obs, illumination=True)
+    finally:
+        lib().orc_set_strip_bf16(0)
     res = dict(obs=obs, pred=pred, grad_vp=P.crop(out["grad_vp"]).sum(0),
                grad_wavelet=out["grad_wavelet"].sum((0, 1)),
                S=P.crop(out["source_illumination"]).sum(0), R=P.crop(out["receiver_illumination"]).sum(0))
@@ -96,7 +101,7 @@ RECORD_BAR = {("c1_order4", "obs"): 6e-5}
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("mode", ["full", "bs", "ckpt"])
+@pytest.mark.parametrize("mode", ["full", "bs", "ckpt", "bs_bf16"])
 def test_matches_oracle(name, mode):
     dev = _dev()
     print(f"[acoustic2d] case={name} mode={mode}", flush=True)
@@ -131,10 +136,16 @@ def test_memory_modes_agree():
     full = _run(z, "full", dev)
     bs = _run(z, "bs", dev)
     ck = _run(z, "ckpt", dev)
+    bf = _run(z, "bs_bf16", dev)
     assert np.array_equal(full["pred"], bs["pred"]) and np.array_equal(full["pred"], ck["pred"])
+    assert np.array_equal(full["pred"], bf["pred"])
     assert np.array_equal(full["grad_wavelet"], bs["grad_wavelet"])
     assert np.array_equal(full["grad_vp"], ck["grad_vp"])          # the replay is the forward, bit for bit
     assert rel_l2(bs["grad_vp"], full["grad_vp"]) <= 1e-4          # the reconstruction is not
+    # bf16 strips (8 mantissa bits) perturb the reconstruction, not the adjoint
+    e = rel_l2(bf["grad_vp"], bs["grad_vp"])
+    print(f"[acoustic2d] bf16 strips: grad_vp rel-L2 vs fp32 strips {e:.2e}", flush=True)
+    assert 0 < e <= 1e-2
 
 
 @pytest.mark.parametrize("slots", [1, 3, 16, 400])

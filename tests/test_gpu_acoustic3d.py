@@ -30,7 +30,8 @@ def _memory(name):
     from sweep_amd.options import BoundarySaving, Ckpt, Full
     if name.startswith("binomial"):   # "binomial<k>": Ckpt(mode='recursive', count=k)
         return Ckpt(mode="recursive", count=int(name[len("binomial"):]))
-    return {"full": Full(), "bs": BoundarySaving(), "ckpt": Ckpt(chunks=5)}[name]
+    return {"full": Full(), "bs": BoundarySaving(), "ckpt": Ckpt(chunks=5),
+            "bs_bf16": BoundarySaving(storage_dtype="bf16")}[name]
 
 
 def _prop(order, shape, dh, dt, abcn, mode, dev, illumination=False):
@@ -112,12 +113,16 @@ def _random_case(order, seed, B=2, nsrc=1, shape=(13, 11, 15), abcn=7, nt=110):
 
 
 def _oracle_run(c, mode, illumination=False):
-    from oracle.oracle import A3DProblem
+    from oracle.oracle import A3DProblem, lib
     P = A3DProblem(c["order"], c["shape"], c["dh"], c["dt"], c["nt"], abcn=c["abcn"], B=c["B"])
     src = c["src"] if c["src"].ndim == 3 else c["src"][:, None, :]
     wav = c["wav"][None] if c["src"].ndim == 3 else c["wav"][:, None, :]
-    o = P.gradient({"full": 0, "bs": 1, "ckpt": 0}[mode], P.pad_model(c["vp"]), wav, P.shift(src), P.shift(c["rec"]),
-                   c["res"], illumination=illumination)
+    lib().orc_set_strip_bf16(1 if mode == "bs_bf16" else 0)
+    try:
+        o = P.gradient({"full": 0, "bs": 1, "ckpt": 0, "bs_bf16": 1}[mode], P.pad_model(c["vp"]), wav, P.shift(src),
+                       P.shift(c["rec"]), c["res"], illumination=illumination)
+    finally:
+        lib().orc_set_strip_bf16(0)
     out = dict(rec=o["rec"], grad_vp=P.crop(o["grad_vp"]).sum(0),
                grad_wavelet=o["grad_wavelet"][0] if c["src"].ndim == 3 else o["grad_wavelet"][:, 0])
     if illumination:
@@ -126,7 +131,7 @@ def _oracle_run(c, mode, illumination=False):
 
 
 @pytest.mark.parametrize("order", [2, 4, 6, 8, 10])
-@pytest.mark.parametrize("mode", ["full", "bs", "ckpt"])
+@pytest.mark.parametrize("mode", ["full", "bs", "ckpt", "bs_bf16"])
 def test_matches_oracle(order, mode):
     dev = _dev()
     print(f"[acoustic3d] oracle order={order} mode={mode}", flush=True)
@@ -136,9 +141,30 @@ def test_matches_oracle(order, mode):
     ref = _oracle_run(c, mode)
     errs = {k: rel_l2(got[k], ref[k]) for k in ("rec", "grad_vp", "grad_wavelet")}
     print(f"[acoustic3d] order {order} {mode} {errs}", flush=True)
+    # bf16 strips: a strip value whose float32 differs from the oracle's by an
+    # ulp can round to the neighbouring bfloat16 (2^-8 apart), so the gradients
+    # carry a few such flips; the bar stays well under the bf16-vs-fp32 effect
+    gbar = 3e-4 if mode == "bs_bf16" else 1e-4
     assert errs["rec"] <= 1e-5
-    assert errs["grad_vp"] <= 1e-4
-    assert errs["grad_wavelet"] <= 1e-4
+    assert errs["grad_vp"] <= gbar
+    assert errs["grad_wavelet"] <= gbar
+
+
+def test_bf16_strips_effect():
+    """bf16 strips halve the tape and move the gradient by the bf16 rounding
+    of the saved boundary only: records unchanged, model gradient within 5e-2
+    of the fp32-strip one (and not equal to it).  On this 13 x 11 x 15 box the
+    strips are most of the physical cells; 1.2% was measured."""
+    dev = _dev()
+    c = _random_case(8, seed=12)
+    args = (c["order"], c["shape"], c["dh"], c["dt"], c["abcn"])
+    io = (c["vp"], c["wav"], c["src"], c["rec"], c["res"], dev)
+    bs = _gpu_run(*args, "bs", *io)
+    bf = _gpu_run(*args, "bs_bf16", *io)
+    assert np.array_equal(bs["rec"], bf["rec"])
+    e = rel_l2(bf["grad_vp"], bs["grad_vp"])
+    print(f"[acoustic3d] bf16 strips: grad_vp rel-L2 vs fp32 strips {e:.2e}", flush=True)
+    assert 0 < e <= 5e-2
 
 
 def test_several_sources_per_shot():

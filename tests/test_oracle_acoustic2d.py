@@ -200,3 +200,22 @@ def test_fma_contraction_sensitivity_of_c1():
         gaps[name] = rel_l2(b, a)
     assert gaps["c1_order4"] > 1e-5
     assert gaps["het_order8"] < 5e-6
+
+
+def test_bf16_strip_rounding_is_round_to_nearest_even():
+    """The oracle's strip rounding against torch's float32 -> bfloat16
+    conversion (round to nearest, ties to even): ties in both directions,
+    subnormals, overflow to inf, signed zeros, inf and a NaN."""
+    import torch
+    from oracle.oracle import lib
+    L = lib()
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([
+        rng.standard_normal(2000).astype(np.float32) * np.float32(10.0) ** rng.integers(-30, 30, 2000),
+        np.array([1.0, 1.00390625, 1.005859375, 1.0078125, 1.01171875, -1.00390625, 1e-40, -1e-41, 3.39e38,
+                  3.4e38, np.inf, -np.inf, 0.0, -0.0], dtype=np.float32)]).astype(np.float32)
+    ref = torch.tensor(xs).to(torch.bfloat16).to(torch.float32).numpy()
+    got = np.array([L.orc_bf16_round(float(x)) for x in xs], dtype=np.float32)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    assert np.isnan(L.orc_bf16_round(float("nan")))
+    assert L.orc_bf16_round(1.00390625) == 1.0 and L.orc_bf16_round(1.01171875) == 1.015625   # ties to even
