@@ -273,6 +273,7 @@ def main():
         sys.exit(2)
     import torch
     from sweep_amd import _core
+    from sweep_amd.loss import l2_misfit
     W = WORKLOADS[args.workload]
     nt = args.nt or W["nt"]
     per_launch = args.shots_per_launch or W["per_launch"]
@@ -298,17 +299,14 @@ def main():
     with torch.no_grad():
         obs = [solver(wav, src[a:b], rec[a:b], models=vt) for a, b in batches]
     del vt
-    masks = [None if mask is None else torch.tensor(mask[a:b], device=dev)[:, None, :, None] for a, b in batches]
+    masks = [None if mask is None else torch.tensor(mask[a:b], device=dev) for a, b in batches]
     m0 = [torch.tensor(m, device=dev) for m in vp_init]
 
     def step():
         ms = [m.clone().requires_grad_(True) for m in m0]
         for k, (a, b) in enumerate(batches):
             pred = solver(wav, src[a:b], rec[a:b], models=ms)
-            r = pred - obs[k]
-            if masks[k] is not None:
-                r = r * masks[k]
-            loss = 0.5 * r.pow(2).sum()
+            loss = l2_misfit(pred, obs[k], masks[k])
             loss.backward()
         return ms[0].grad
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWP_ABI_VERSION 3
+#define SWP_ABI_VERSION 4
 
 /* Memory strategy of a gradient run (reference: propagator/options.py
  * Full / BoundarySaving(storage='gpu') / Ckpt(mode='chunk'|'recursive')). */
@@ -162,6 +162,31 @@ SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launche
  * propagator/_c.py:2608-2615); no C function of the reference corresponds. */
 SWP_CAPI int64_t swp_revolve_schedule(int32_t nt, int32_t slots, int32_t *ops, int64_t cap);
 SWP_CAPI int64_t swp_revolve_spine(int32_t nt, int32_t slots, int32_t *pts, int64_t cap);
+
+/* Record wire format.  A forward entry writes its record raw, time innermost:
+ * (nfield, B, nrec, nt); callers and losses use the canonical
+ * (B, nt, nrec, nfield).  swp_record_to_canonical replaces the reference's
+ * _cuda_record_to_canonical (propagator/_c.py:109-122), swp_record_to_raw its
+ * _canonical_to_cuda_record (_c.py:158-186) for the adjoint source.  Device
+ * pointers, not overlapping; 1 <= nfield <= 5; launched on the thread's
+ * stream.  Returns 0, or 1 with a message in err. */
+SWP_CAPI int swp_record_to_canonical(const float *raw, float *canonical, int32_t nfield, int32_t B,
+                                     int32_t nrec, int32_t nt, char *err, int err_cap);
+SWP_CAPI int swp_record_to_raw(const float *canonical, float *raw, int32_t nfield, int32_t B,
+                               int32_t nrec, int32_t nt, char *err, int err_cap);
+
+/* L2 misfit over canonical records: residual = (pred - obs) * mask[b, r]
+ * (mask (B, nrec) or NULL for none, residual NULL to skip it) and
+ * loss[0] = 0.5 * sum residual^2, accumulated in float64 in a fixed order
+ * (the same value on every run).  partials: device scratch of
+ * SWP_MISFIT_PARTIALS doubles.  residual is the adjoint source of the loss.
+ * The reference leaves losses to a separate package (sweep_loss, not part of
+ * it); this is the loss of its tests and FD checks, 0.5 * ||pred - obs||^2
+ * with padded receivers masked out. */
+#define SWP_MISFIT_PARTIALS 1024
+SWP_CAPI int swp_l2_misfit(const float *pred, const float *obs, const float *mask, int32_t nfield, int32_t B,
+                           int32_t nrec, int32_t nt, float *residual, double *loss, double *partials,
+                           char *err, int err_cap);
 
 #ifdef __cplusplus
 }

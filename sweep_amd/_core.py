@@ -15,7 +15,8 @@ import threading
 from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libswp_core.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
+MISFIT_PARTIALS = 1024   # SWP_MISFIT_PARTIALS
 
 _F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.POINTER(ctypes.c_int32)
@@ -69,7 +70,8 @@ EXPORTS = (
     "swp_core_abi_version", "swp_layout_count", "swp_layout_probe", "swp_entry_count",
     "swp_entry_name", "swp_entry_kind", "swp_entry_available", "swp_core_device", "swp_query",
     "swp_call", "swp_set_stream", "swp_get_stream", "swp_set_timing", "swp_last_kernel_stats",
-    "swp_revolve_schedule", "swp_revolve_spine",
+    "swp_revolve_schedule", "swp_revolve_spine", "swp_record_to_canonical", "swp_record_to_raw",
+    "swp_l2_misfit",
 )
 
 
@@ -107,6 +109,13 @@ def _declare(L):
     for f in (L.swp_revolve_schedule, L.swp_revolve_spine):
         f.restype = ctypes.c_int64
         f.argtypes = [ctypes.c_int32, ctypes.c_int32, _I32, ctypes.c_int64]
+    _i = ctypes.c_int32
+    for f in (L.swp_record_to_canonical, L.swp_record_to_raw):
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _i, _i, _i, _i, ctypes.c_char_p, ctypes.c_int]
+    L.swp_l2_misfit.restype = ctypes.c_int
+    L.swp_l2_misfit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _i, _i, _i, _i,
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
 
 
 def load():
@@ -192,6 +201,18 @@ def kernel_stats(which: int):
     if L.swp_last_kernel_stats(which, ctypes.byref(ms), ctypes.byref(n)) != 0:
         raise RuntimeError("swp_last_kernel_stats failed")
     return float(ms.value), int(n.value)
+
+
+def on_stream(device, fn, *args):
+    """fn(*args, err, cap) on torch's current stream of ``device``; raises on
+    a non-zero return."""
+    import torch
+    L = load()
+    err = ctypes.create_string_buffer(1024)
+    with torch.cuda.device(device):
+        L.swp_set_stream(ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream))
+        rc = fn(*args, err, 1024)
+    _raise(rc, err, fn.__name__)
 
 
 def _pairs(fn, nt: int, slots: int):

@@ -11,6 +11,7 @@
 #include "acoustic2d.h"
 #include "acoustic3d.h"
 #include "elastic2d.h"
+#include "records.h"
 #include "revolve.h"
 
 namespace swp {
@@ -260,6 +261,27 @@ SWP_CAPI int64_t swp_revolve_spine(int32_t nt, int32_t slots, int32_t *pts, int6
     } catch (...) {
         return -1;
     }
+}
+
+SWP_CAPI int swp_record_to_canonical(const float *raw, float *canonical, int32_t nfield, int32_t B,
+                                     int32_t nrec, int32_t nt, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { rec::to_canonical(raw, canonical, nfield, B, nrec, nt); });
+}
+
+SWP_CAPI int swp_record_to_raw(const float *canonical, float *raw, int32_t nfield, int32_t B, int32_t nrec,
+                               int32_t nt, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { rec::to_raw(canonical, raw, nfield, B, nrec, nt); });
+}
+
+SWP_CAPI int swp_l2_misfit(const float *pred, const float *obs, const float *mask, int32_t nfield, int32_t B,
+                           int32_t nrec, int32_t nt, float *residual, double *loss, double *partials,
+                           char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] {
+        rec::l2_misfit(pred, obs, mask, nfield, B, nrec, nt, residual, loss, partials);
+    });
 }
 
 }  // extern "C"

@@ -29,7 +29,7 @@ import numpy as np
 import torch
 from torch.autograd.function import once_differentiable
 
-from . import _core, geometry, pml, scalars
+from . import _core, geometry, pml, records, scalars
 from .equations import Acoustic, Acoustic3D, Elastic, WaveEquation
 from .options import BoundarySaving, Ckpt, Full, as_memory_strategy
 
@@ -85,9 +85,7 @@ class _Wrapper(torch.autograd.Function):
         ctx.tape = tape
         ctx.used = False
         ctx.save_for_backward(wavelet, *models)
-        if call.nfield > 1:
-            return record.permute(1, 3, 2, 0).contiguous()
-        return record.permute(0, 2, 1).unsqueeze(-1).contiguous()
+        return records.to_canonical(record)
 
     @staticmethod
     @once_differentiable
@@ -103,11 +101,7 @@ class _Wrapper(torch.autograd.Function):
         dev = call.device
         if grad is None:
             return (None, None) + (None,) * len(models)
-        if call.nfield > 1:
-            adj = grad.permute(3, 0, 2, 1).contiguous()
-        else:
-            adj = grad[..., 0].permute(0, 2, 1).contiguous()
-        adj = adj.to(torch.float32)
+        adj = records.to_raw(grad.to(torch.float32), 4 if call.nfield > 1 else 3)
         entry = {_core.MEM_FULL: 1, _core.MEM_BOUNDARY: 2, _core.MEM_CKPT: 3, _core.MEM_BINOMIAL: 4}[P.memory]
         name = call.names[entry]
         if call.illumination:
