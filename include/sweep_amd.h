@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWP_ABI_VERSION 4
+#define SWP_ABI_VERSION 5
 
 /* Memory strategy of a gradient run (reference: propagator/options.py
  * Full / BoundarySaving(storage='gpu') / Ckpt(mode='chunk'|'recursive')). */
@@ -187,6 +187,54 @@ SWP_CAPI int swp_record_to_raw(const float *canonical, float *raw, int32_t nfiel
 SWP_CAPI int swp_l2_misfit(const float *pred, const float *obs, const float *mask, int32_t nfield, int32_t B,
                            int32_t nrec, int32_t nt, float *residual, double *loss, double *partials,
                            char *err, int err_cap);
+
+/* ---- Learned preconditioner (2-D U-Net) kernels: no counterpart in the
+ * reference, whose path has no learned component.  Tensors are NHWC views:
+ * a device pointer to channel 0 of pixel 0 and a channel stride; bf16 tensors
+ * are uint16 bit patterns (round to nearest even).  Convolutions run on the
+ * bf16 matrix cores with float32 accumulation, same-size, ks 1 or 3; channel
+ * counts, strides and pointers of bf16 views are multiples of 8 elements /
+ * 16 bytes.  Weights are [cout][ks*ks][cin].  All launch on the thread's
+ * stream and return 0, or 1 with a message. */
+SWP_CAPI int swp_nn_conv_fwd(const uint16_t *in, int32_t N, int32_t H, int32_t W, int32_t cin, int32_t in_stride,
+                             const uint16_t *w, int32_t cout, int32_t ks, const float *bias, int32_t relu,
+                             void *out, int32_t out_stride, int32_t out_f32, char *err, int err_cap);
+/* Float count of the scratch swp_nn_conv_wgrad needs for these extents. */
+SWP_CAPI int64_t swp_nn_conv_wgrad_scratch(int32_t N, int32_t H, int32_t W, int32_t cin, int32_t cout, int32_t ks);
+/* dw[co][tap][ci] = sum_p dout[p][co] in[p + off(tap)][ci] (float32,
+ * overwritten, fixed summation order); db[co] = sum_p dout[p][co] unless NULL. */
+SWP_CAPI int swp_nn_conv_wgrad(const uint16_t *in, int32_t N, int32_t H, int32_t W, int32_t cin, int32_t in_stride,
+                               const uint16_t *dout, int32_t cout, int32_t dout_stride, int32_t ks, float *dw,
+                               float *db, float *scratch, char *err, int err_cap);
+/* wt[ci][tap][co] = w[co][ks*ks-1-tap][ci]: the data gradient of a
+ * convolution is swp_nn_conv_fwd(dout, wt) with cin and cout swapped. */
+SWP_CAPI int swp_nn_conv_flip(const uint16_t *w, int32_t cout, int32_t cin, int32_t ks, uint16_t *wt,
+                              char *err, int err_cap);
+SWP_CAPI int swp_nn_relu_bwd(uint16_t *d, int32_t d_stride, const uint16_t *act, int32_t act_stride, int64_t P,
+                             int32_t C, char *err, int err_cap);
+/* 2x2 average pool of an (N, H, W, C) view; the adjoint adds into din. */
+SWP_CAPI int swp_nn_pool2_fwd(const uint16_t *in, int32_t in_stride, int32_t N, int32_t H, int32_t W, int32_t C,
+                              uint16_t *out, int32_t out_stride, char *err, int err_cap);
+SWP_CAPI int swp_nn_pool2_bwd_acc(const uint16_t *dout, int32_t dout_stride, int32_t N, int32_t H, int32_t W,
+                                  int32_t C, uint16_t *din, int32_t din_stride, char *err, int err_cap);
+/* Nearest 2x upsample of an (N, H, W, C) view and its adjoint (overwrites din). */
+SWP_CAPI int swp_nn_up2_fwd(const uint16_t *in, int32_t in_stride, int32_t N, int32_t H, int32_t W, int32_t C,
+                            uint16_t *out, int32_t out_stride, char *err, int err_cap);
+SWP_CAPI int swp_nn_up2_bwd(const uint16_t *dout, int32_t dout_stride, int32_t N, int32_t H, int32_t W, int32_t C,
+                            uint16_t *din, int32_t din_stride, char *err, int err_cap);
+/* x[p][c] = bf16(scale[c] * planes[c][p]) for c < nsrc (<= 4), 0 up to stride. */
+SWP_CAPI int swp_nn_pack_input(const float *const *planes, const float *scale, int32_t nsrc, int64_t P, uint16_t *x,
+                               int32_t stride, char *err, int err_cap);
+/* loss[0] = sum_p (out[p][0] - target[p])^2 mask[p] / count (float64, fixed
+ * order; mask NULL: 1); dout[p][0] = bf16(2 (out - target) mask / count), the
+ * other channels of dout zero.  partials: SWP_MISFIT_PARTIALS doubles. */
+SWP_CAPI int swp_nn_mse(const float *out, int32_t out_stride, const float *target, const float *mask, int64_t P,
+                        double count, uint16_t *dout, int32_t dout_stride, double *loss, double *partials,
+                        char *err, int err_cap);
+/* One Adam step (float32 master weights; wbf, if not NULL, gets their bf16). */
+SWP_CAPI int swp_nn_adam(float *w, const float *g, float *m, float *v, uint16_t *wbf, int64_t n, float lr, float b1,
+                         float b2, float eps, int32_t step, char *err, int err_cap);
+SWP_CAPI int swp_nn_to_bf16(const float *src, uint16_t *dst, int64_t n, char *err, int err_cap);
 
 #ifdef __cplusplus
 }

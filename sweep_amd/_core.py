@@ -15,7 +15,7 @@ import threading
 from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libswp_core.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 MISFIT_PARTIALS = 1024   # SWP_MISFIT_PARTIALS
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -72,6 +72,9 @@ EXPORTS = (
     "swp_call", "swp_set_stream", "swp_get_stream", "swp_set_timing", "swp_last_kernel_stats",
     "swp_revolve_schedule", "swp_revolve_spine", "swp_record_to_canonical", "swp_record_to_raw",
     "swp_l2_misfit",
+    "swp_nn_conv_fwd", "swp_nn_conv_wgrad_scratch", "swp_nn_conv_wgrad", "swp_nn_conv_flip", "swp_nn_relu_bwd",
+    "swp_nn_pool2_fwd", "swp_nn_pool2_bwd_acc", "swp_nn_up2_fwd", "swp_nn_up2_bwd", "swp_nn_pack_input",
+    "swp_nn_mse", "swp_nn_adam", "swp_nn_to_bf16",
 )
 
 
@@ -113,6 +116,28 @@ def _declare(L):
     for f in (L.swp_record_to_canonical, L.swp_record_to_raw):
         f.restype = ctypes.c_int
         f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _i, _i, _i, _i, ctypes.c_char_p, ctypes.c_int]
+    _p, _e, _c = ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int
+    _i64, _f, _d = ctypes.c_int64, ctypes.c_float, ctypes.c_double
+    nn_sigs = {
+        "swp_nn_conv_fwd": [_p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _i, _p, _i, _i],
+        "swp_nn_conv_wgrad": [_p, _i, _i, _i, _i, _i, _p, _i, _i, _i, _p, _p, _p],
+        "swp_nn_conv_flip": [_p, _i, _i, _i, _p],
+        "swp_nn_relu_bwd": [_p, _i, _p, _i, _i64, _i],
+        "swp_nn_pool2_fwd": [_p, _i, _i, _i, _i, _i, _p, _i],
+        "swp_nn_pool2_bwd_acc": [_p, _i, _i, _i, _i, _i, _p, _i],
+        "swp_nn_up2_fwd": [_p, _i, _i, _i, _i, _i, _p, _i],
+        "swp_nn_up2_bwd": [_p, _i, _i, _i, _i, _i, _p, _i],
+        "swp_nn_pack_input": [_p, _p, _i, _i64, _p, _i],
+        "swp_nn_mse": [_p, _i, _p, _p, _i64, _d, _p, _i, _p, _p],
+        "swp_nn_adam": [_p, _p, _p, _p, _p, _i64, _f, _f, _f, _f, _i],
+        "swp_nn_to_bf16": [_p, _p, _i64],
+    }
+    for name, args in nn_sigs.items():
+        f = getattr(L, name)
+        f.restype = _c
+        f.argtypes = args + [_e, _c]
+    L.swp_nn_conv_wgrad_scratch.restype = _i64
+    L.swp_nn_conv_wgrad_scratch.argtypes = [_i, _i, _i, _i, _i, _i]
     L.swp_l2_misfit.restype = ctypes.c_int
     L.swp_l2_misfit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _i, _i, _i, _i,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]

@@ -13,6 +13,7 @@
 #include "elastic2d.h"
 #include "records.h"
 #include "revolve.h"
+#include "unet.h"
 
 namespace swp {
 
@@ -282,6 +283,95 @@ SWP_CAPI int swp_l2_misfit(const float *pred, const float *obs, const float *mas
     return guarded(err, err_cap, [&] {
         rec::l2_misfit(pred, obs, mask, nfield, B, nrec, nt, residual, loss, partials);
     });
+}
+
+SWP_CAPI int swp_nn_conv_fwd(const uint16_t *in, int32_t N, int32_t H, int32_t W, int32_t cin, int32_t in_stride,
+                             const uint16_t *w, int32_t cout, int32_t ks, const float *bias, int32_t relu,
+                             void *out, int32_t out_stride, int32_t out_f32, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] {
+        nn::conv_fwd(in, N, H, W, cin, in_stride, w, cout, ks, bias, relu != 0, out, out_stride, out_f32 != 0);
+    });
+}
+
+SWP_CAPI int64_t swp_nn_conv_wgrad_scratch(int32_t N, int32_t H, int32_t W, int32_t cin, int32_t cout, int32_t ks)
+{
+    if (N < 1 || H < 1 || W < 1 || cin < 1 || cout < 1 || (ks != 1 && ks != 3)) return -1;
+    return nn::conv_wgrad_partials(N, H, W, cin, cout, ks);
+}
+
+SWP_CAPI int swp_nn_conv_wgrad(const uint16_t *in, int32_t N, int32_t H, int32_t W, int32_t cin, int32_t in_stride,
+                               const uint16_t *dout, int32_t cout, int32_t dout_stride, int32_t ks, float *dw,
+                               float *db, float *scratch, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] {
+        nn::conv_wgrad(in, N, H, W, cin, in_stride, dout, cout, dout_stride, ks, dw, db, scratch);
+    });
+}
+
+SWP_CAPI int swp_nn_conv_flip(const uint16_t *w, int32_t cout, int32_t cin, int32_t ks, uint16_t *wt, char *err,
+                              int err_cap)
+{
+    return guarded(err, err_cap, [&] { nn::conv_flip(w, cout, cin, ks, wt); });
+}
+
+SWP_CAPI int swp_nn_relu_bwd(uint16_t *d, int32_t d_stride, const uint16_t *act, int32_t act_stride, int64_t P,
+                             int32_t C, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { nn::relu_bwd(d, d_stride, act, act_stride, P, C); });
+}
+
+SWP_CAPI int swp_nn_pool2_fwd(const uint16_t *in, int32_t in_stride, int32_t N, int32_t H, int32_t W, int32_t C,
+                              uint16_t *out, int32_t out_stride, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { nn::pool2_fwd(in, in_stride, N, H, W, C, out, out_stride); });
+}
+
+SWP_CAPI int swp_nn_pool2_bwd_acc(const uint16_t *dout, int32_t dout_stride, int32_t N, int32_t H, int32_t W,
+                                  int32_t C, uint16_t *din, int32_t din_stride, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { nn::pool2_bwd_acc(dout, dout_stride, N, H, W, C, din, din_stride); });
+}
+
+SWP_CAPI int swp_nn_up2_fwd(const uint16_t *in, int32_t in_stride, int32_t N, int32_t H, int32_t W, int32_t C,
+                            uint16_t *out, int32_t out_stride, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { nn::up2_fwd(in, in_stride, N, H, W, C, out, out_stride); });
+}
+
+SWP_CAPI int swp_nn_up2_bwd(const uint16_t *dout, int32_t dout_stride, int32_t N, int32_t H, int32_t W, int32_t C,
+                            uint16_t *din, int32_t din_stride, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { nn::up2_bwd(dout, dout_stride, N, H, W, C, din, din_stride); });
+}
+
+SWP_CAPI int swp_nn_pack_input(const float *const *planes, const float *scale, int32_t nsrc, int64_t P, uint16_t *x,
+                               int32_t stride, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] {
+        SWP_CHECK(planes && scale, "pack_input: null plane or scale array");
+        nn::pack_input(planes, scale, nsrc, P, x, stride);
+    });
+}
+
+SWP_CAPI int swp_nn_mse(const float *out, int32_t out_stride, const float *target, const float *mask, int64_t P,
+                        double count, uint16_t *dout, int32_t dout_stride, double *loss, double *partials,
+                        char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] {
+        nn::mse(out, out_stride, target, mask, P, count, dout, dout_stride, loss, partials);
+    });
+}
+
+SWP_CAPI int swp_nn_adam(float *w, const float *g, float *m, float *v, uint16_t *wbf, int64_t n, float lr, float b1,
+                         float b2, float eps, int32_t step, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { nn::adam(w, g, m, v, wbf, n, lr, b1, b2, eps, step); });
+}
+
+SWP_CAPI int swp_nn_to_bf16(const float *src, uint16_t *dst, int64_t n, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { nn::to_bf16(src, dst, n); });
 }
 
 }  // extern "C"
