@@ -104,6 +104,7 @@ class PlainFWI:
         self.max_halvings = max_halvings
         self.max_doublings = max_doublings
         self.history: list[IterationRecord] = []
+        self.last = None   # (J, g, S) of the last iteration's starting model
 
     # -- objective -------------------------------------------------------
     def misfit(self, vp: torch.Tensor) -> float:
@@ -151,6 +152,7 @@ class PlainFWI:
         torch.cuda.synchronize(vp.device)
         t0 = time.perf_counter()
         J0, g, S = self.gradient(vp)
+        self.last = (J0, g, S)
         d = self.direction(g, S)
         res = line_search(lambda s: self.misfit(self.trial(vp, d, s)), J0, self.next_step,
                           self.max_halvings, self.max_doublings)

@@ -270,3 +270,63 @@ class UNet:
         self.backward(dout)
         self.adam_step(lr)
         return loss
+
+
+def conv_flops(net: UNet, N: int, H: int, W: int, training: bool = True) -> float:
+    """Multiply-add flops (2 per MAC) of the convolutions of one forward (and,
+    with training, the data and weight gradients; the first layer has no data
+    gradient)."""
+    total = 0.0
+    for name, lay in net.layers.items():
+        lvl = int(name[1]) if name[0] in "ed" else 0
+        P = N * (H >> lvl) * (W >> lvl)
+        f = 2.0 * P * lay.cin * lay.cout * lay.ks * lay.ks
+        total += f
+        if training:
+            total += f * (1 if name == "e0a" else 2)
+    return total
+
+
+def _bench():
+    """Training-step time of the U-Net at the 2-D workload's grid."""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--N", type=int, default=2)
+    ap.add_argument("--H", type=int, default=384)
+    ap.add_argument("--W", type=int, default=1152)
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--base", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=10)
+    args = ap.parse_args()
+    dev = torch.device("cuda:0")
+    net = UNet(dev, levels=args.levels, base=args.base, nsrc=3)
+    g = torch.Generator(device=dev).manual_seed(0)
+    planes = [torch.randn((args.N, args.H, args.W), device=dev, generator=g) for _ in range(3)]
+    x, _ = net.input(planes, [1.0, 1.0, 1.0])
+    target = torch.randn((args.N, args.H, args.W), device=dev, generator=g)
+    for _ in range(2):
+        net.train_step(x, target)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.train_step(x, target)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    fl = conv_flops(net, args.N, args.H, args.W)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.forward(x)
+    torch.cuda.synchronize()
+    ms_f = (time.perf_counter() - t0) / args.steps * 1e3
+    ff = conv_flops(net, args.N, args.H, args.W, training=False)
+    print(json.dumps({"N": args.N, "H": args.H, "W": args.W, "levels": args.levels, "base": args.base,
+                      "train_step_ms": round(ms, 3), "train_conv_tflops": round(fl / ms * 1e-9, 2),
+                      "forward_ms": round(ms_f, 3), "forward_conv_tflops": round(ff / ms_f * 1e-9, 2),
+                      "conv_gflop_per_train_step": round(fl * 1e-9, 2)}), flush=True)
+
+
+if __name__ == "__main__":
+    _bench()
