@@ -121,7 +121,8 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t *__restric
 // Block: 64 (co) x 64 (ci) for one tap over one pixel split; wave wv owns
 // the 32 x 32 sub-tile (co (wv / 2) * 32, ci (wv % 2) * 32).
 // Grid: (ceil(cin / 64), ceil(cout / 64), ks * ks * splits).
-constexpr int kLP = 40;   // LDS row pitch (bf16): 32 pixels + 8, 16-byte aligned rows
+constexpr int kCP = 64;   // pixels per LDS chunk (two 32-pixel MFMA k-steps)
+constexpr int kLP = 72;   // LDS row pitch (bf16): kCP pixels + 8, 16-byte aligned rows
 
 template <int KS>
 __global__ void __launch_bounds__(256) wgrad_kernel(const uint16_t *__restrict__ in, int H, int W, int cin, int is,
@@ -145,35 +146,42 @@ __global__ void __launch_bounds__(256) wgrad_kernel(const uint16_t *__restrict__
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int64_t p0 = pb; p0 < pe; p0 += 32) {
-        const int64_t p = p0 + lp;
-        u32x4 vd = {0u, 0u, 0u, 0u}, vi = {0u, 0u, 0u, 0u};
-        if (p < pe) {
-            if (co0 + lc < cout) vd = *reinterpret_cast<const u32x4 *>(dout + p * ds + co0 + lc);
-            if (ci0 + lc < cin) {
-                const int64_t nn = p / HW, rem = p % HW;
-                const int sy = int(rem / W) + dy, sx = int(rem % W) + dx;
-                if (sy >= 0 && sy < H && sx >= 0 && sx < W)
-                    vi = *reinterpret_cast<const u32x4 *>(in + ((nn * H + sy) * W + sx) * is + ci0 + lc);
+    for (int64_t p0 = pb; p0 < pe; p0 += kCP) {
+#pragma unroll
+        for (int h = 0; h < kCP / 32; ++h) {
+            const int pl = lp + 32 * h;
+            const int64_t p = p0 + pl;
+            u32x4 vd = {0u, 0u, 0u, 0u}, vi = {0u, 0u, 0u, 0u};
+            if (p < pe) {
+                if (co0 + lc < cout) vd = *reinterpret_cast<const u32x4 *>(dout + p * ds + co0 + lc);
+                if (ci0 + lc < cin) {
+                    const int64_t nn = p / HW, rem = p % HW;
+                    const int sy = int(rem / W) + dy, sx = int(rem % W) + dx;
+                    if (sy >= 0 && sy < H && sx >= 0 && sx < W)
+                        vi = *reinterpret_cast<const u32x4 *>(in + ((nn * H + sy) * W + sx) * is + ci0 + lc);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                sD[(lc + 2 * e) * kLP + pl] = uint16_t(vd[e] & 0xffffu);
+                sD[(lc + 2 * e + 1) * kLP + pl] = uint16_t(vd[e] >> 16);
+                sI[(lc + 2 * e) * kLP + pl] = uint16_t(vi[e] & 0xffffu);
+                sI[(lc + 2 * e + 1) * kLP + pl] = uint16_t(vi[e] >> 16);
             }
         }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            sD[(lc + 2 * e) * kLP + lp] = uint16_t(vd[e] & 0xffffu);
-            sD[(lc + 2 * e + 1) * kLP + lp] = uint16_t(vd[e] >> 16);
-            sI[(lc + 2 * e) * kLP + lp] = uint16_t(vi[e] & 0xffffu);
-            sI[(lc + 2 * e + 1) * kLP + lp] = uint16_t(vi[e] >> 16);
-        }
         __syncthreads();
-        bf16x8 A[2], Bv[2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) A[a] = ld8(sD + (wco + 16 * a + r16) * kLP + 8 * g);
+        for (int k = 0; k < kCP / 32; ++k) {
+            bf16x8 A[2], Bv[2];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) Bv[b] = ld8(sI + (wci + 16 * b + r16) * kLP + 8 * g);
+            for (int a = 0; a < 2; ++a) A[a] = ld8(sD + (wco + 16 * a + r16) * kLP + 32 * k + 8 * g);
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) Bv[b] = ld8(sI + (wci + 16 * b + r16) * kLP + 32 * k + 8 * g);
 #pragma unroll
-            for (int b = 0; b < 2; ++b) acc[a][b] = mma(A[a], Bv[b], acc[a][b]);
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = mma(A[a], Bv[b], acc[a][b]);
+        }
         __syncthreads();
     }
     float *dst = partials + (int64_t(split) * (KS * KS) + tap) * cout * cin;
@@ -207,25 +215,47 @@ __global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float *__restri
 }
 
 // partial[split, c] = sum over the split's pixels of x[p, c]; then out[c].
+// Thread t takes channel c0 + t % CB and pixel sub-lane t / CB (J = 256 / CB
+// sub-lanes stride the split's pixels), the J lane sums meet in LDS.
 __global__ void __launch_bounds__(256) colsum_kernel(const uint16_t *__restrict__ x, int xs, int C, int64_t P,
                                                      int64_t per, float *__restrict__ partial)
 {
+    __shared__ float red[256];
     const int64_t pb = int64_t(blockIdx.x) * per, pe = std::min(P, pb + per);
-    for (int c = int(threadIdx.x); c < C; c += 256) {
+    const int CB = C < 256 ? C : 256;
+    const int J = 256 / CB;
+    const int t = int(threadIdx.x), j = t / CB, cl = t % CB;
+    for (int c0 = 0; c0 < C; c0 += CB) {
+        const int c = c0 + cl;
         float s = 0.f;
-        for (int64_t p = pb; p < pe; ++p) s += bf2f(x[p * xs + c]);
-        partial[int64_t(blockIdx.x) * C + c] = s;
+        if (j < J && c < C)
+            for (int64_t p = pb + j; p < pe; p += J) s += bf2f(x[p * xs + c]);
+        red[t] = s;
+        __syncthreads();
+        if (j == 0 && c < C) {
+            float tot = 0.f;
+            for (int k = 0; k < J; ++k) tot += red[k * CB + cl];
+            partial[int64_t(blockIdx.x) * C + c] = tot;
+        }
+        __syncthreads();
     }
 }
 
+// One block per channel: a fixed-order tree over the splits.
 __global__ void __launch_bounds__(256) colsum_reduce_kernel(const float *__restrict__ partial, int splits, int C,
                                                             float *__restrict__ out)
 {
-    for (int c = int(threadIdx.x); c < C; c += 256) {
-        float s = 0.f;
-        for (int sp = 0; sp < splits; ++sp) s += partial[int64_t(sp) * C + c];
-        out[c] = s;
+    __shared__ float red[256];
+    const int c = int(blockIdx.x);
+    float s = 0.f;
+    for (int sp = int(threadIdx.x); sp < splits; sp += 256) s += partial[int64_t(sp) * C + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[c] = red[0];
 }
 
 __global__ void __launch_bounds__(256) flip_kernel(const uint16_t *__restrict__ w, int cout, int cin, int taps,
@@ -414,7 +444,7 @@ int wgrad_splits(int64_t P, int cin, int cout, int ks)
     return int(std::min<int64_t>(s, 64));
 }
 
-constexpr int kColSplits = 64;
+constexpr int kColSplits = 512;
 
 }  // namespace
 
@@ -461,7 +491,7 @@ void conv_wgrad(const uint16_t *in, int N, int H, int W, int cin, int in_stride,
     const int64_t P = int64_t(N) * H * W;
     const int splits = wgrad_splits(P, cin, cout, ks);
     int64_t per = (P + splits - 1) / splits;
-    per = (per + 31) / 32 * 32;
+    per = (per + kCP - 1) / kCP * kCP;
     const dim3 grid(unsigned((cin + 63) / 64), unsigned((cout + 63) / 64), unsigned(ks * ks * splits));
     if (ks == 3)
         hipLaunchKernelGGL(wgrad_kernel<3>, grid, dim3(256), 0, stream(), in, H, W, cin, in_stride, dout, cout,
@@ -480,7 +510,8 @@ void conv_wgrad(const uint16_t *in, int N, int H, int W, int cin, int in_stride,
         hipLaunchKernelGGL(colsum_kernel, dim3(kColSplits), dim3(256), 0, stream(), dout, dout_stride, cout, P, cper,
                            cp);
         SWP_LAUNCHED();
-        hipLaunchKernelGGL(colsum_reduce_kernel, dim3(1), dim3(256), 0, stream(), cp, kColSplits, cout, db);
+        hipLaunchKernelGGL(colsum_reduce_kernel, dim3(unsigned(cout)), dim3(256), 0, stream(), cp, kColSplits, cout,
+                           db);
         SWP_LAUNCHED();
     }
 }
