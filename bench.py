@@ -263,6 +263,9 @@ def main():
     ap.add_argument("--memory", default=None,
                     help="gradient memory strategy: bs | bs_bf16 | full | ckpt:<chunks> | binomial:<slots> "
                          "(default: the workload's)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick shots per launch and the z-march segment heights by timing short probes first "
+                         "(untimed); the choice is reported under config.autotuned")
     ap.add_argument("--pmc-launch", action="store_true",
                     help="counter runs: a 1 GiB device copy (the calibration of FETCH_SIZE / WRITE_SIZE), "
                          "then one untimed gradient launch; prints no JSON")
@@ -293,6 +296,15 @@ def main():
         print(f"[bench] {W['name']} nt={nt}: one launch of {min(per_launch, nshots)} shots done", file=sys.stderr,
               flush=True)
         return
+    autotuned = None
+    if args.autotune:
+        from sweep_amd.autotune import tune
+        print(f"[bench] {W['name']}: autotuning", file=sys.stderr, flush=True)
+        res = tune(solver, wav, src, rec, [torch.tensor(m, device=dev) for m in vp_init], nshots, full_nt=nt)
+        per_launch = res.shots_per_launch
+        autotuned = {"shots_per_launch": per_launch, **res.knobs,
+                     "probe_us_per_shot_step": round(res.seconds_per_shot_step * 1e6, 3)}
+        print(f"[bench] autotuned {autotuned}", file=sys.stderr, flush=True)
     batches = [(b, min(b + per_launch, nshots)) for b in range(0, nshots, per_launch)]
     print(f"[bench] {W['name']} nt={nt}: building obs", file=sys.stderr, flush=True)
     vt = [torch.tensor(m, device=dev) for m in vp_true]
@@ -351,10 +363,11 @@ def main():
                    "grid": list(W["shape"]), "runtime_grid": list(grid), "nt": nt, "shots": nshots,
                    "shots_per_launch": per_launch, "order": W["order"],
                    "memory": args.memory or ("ckpt:100" if elastic else "bs"),
-                   "tape_bytes_per_launch": tape_bytes},
+                   "tape_bytes_per_launch": tape_bytes,
+                   **({"autotuned": autotuned} if autotuned else {})},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, dom, per_launch),
+                     "traffic": None if autotuned else measured_traffic(args.workload, dom, per_launch),
                      "algorithmic_bytes_per_launch": int(abytes[dom]),
                      "avg_launch_ms": per[dom]["avg_ms"],
                      "kernels": {names[c]: {"avg_launch_ms": per[c]["avg_ms"], "GB/s": per[c]["gbs"],

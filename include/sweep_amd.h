@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWP_ABI_VERSION 5
+#define SWP_ABI_VERSION 6
 
 /* Memory strategy of a gradient run (reference: propagator/options.py
  * Full / BoundarySaving(storage='gpu') / Ckpt(mode='chunk'|'recursive')). */
@@ -162,6 +162,25 @@ SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launche
  * propagator/_c.py:2608-2615); no C function of the reference corresponds. */
 SWP_CAPI int64_t swp_revolve_schedule(int32_t nt, int32_t slots, int32_t *ops, int64_t cap);
 SWP_CAPI int64_t swp_revolve_spine(int32_t nt, int32_t slots, int32_t *pts, int64_t cap);
+
+/* Launch-shape tuning (no reference counterpart: the reference fixes its
+ * block shapes at compile time).  key: SWP_TUNE_*; value > 0 sets it, 0
+ * restores the default (the SWP_TZ* environment variable if set, else the
+ * built-in).  Applies to calls made after it, process-wide.  Records are
+ * bit-identical under any setting, and so are 3-D gradients; a 2-D gradient
+ * moves at the float32 rounding level (measured 4e-8 relative) when the
+ * reconstruction segment height changes.  swp_set_tuning returns 0, or 1 for an unknown key or a
+ * negative value; swp_get_tuning returns the value set (0: default), -1 for
+ * an unknown key. */
+enum {
+    SWP_TUNE_TZ2 = 0,        /* 2-D interior z-march segment rows (default 32) */
+    SWP_TUNE_TZ2_BAND = 1,   /* 2-D CPML-band segment rows (default 8) */
+    SWP_TUNE_TZ2_RECON = 2,  /* 2-D reconstruction segment rows (default 16) */
+    SWP_TUNE_TZ3 = 3,        /* 3-D z-march segment planes (default 32) */
+    SWP_TUNE_COUNT = 4
+};
+SWP_CAPI int swp_set_tuning(int32_t key, int32_t value);
+SWP_CAPI int32_t swp_get_tuning(int32_t key);
 
 /* Record wire format.  A forward entry writes its record raw, time innermost:
  * (nfield, B, nrec, nt); callers and losses use the canonical

@@ -15,7 +15,7 @@ import threading
 from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libswp_core.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 MISFIT_PARTIALS = 1024   # SWP_MISFIT_PARTIALS
 
 _F = ctypes.POINTER(ctypes.c_float)
@@ -74,8 +74,9 @@ EXPORTS = (
     "swp_l2_misfit",
     "swp_nn_conv_fwd", "swp_nn_conv_wgrad_scratch", "swp_nn_conv_wgrad", "swp_nn_conv_flip", "swp_nn_relu_bwd",
     "swp_nn_pool2_fwd", "swp_nn_pool2_bwd_acc", "swp_nn_up2_fwd", "swp_nn_up2_bwd", "swp_nn_pack_input",
-    "swp_nn_mse", "swp_nn_adam", "swp_nn_to_bf16",
+    "swp_nn_mse", "swp_nn_adam", "swp_nn_to_bf16", "swp_set_tuning", "swp_get_tuning",
 )
+TUNE_TZ2, TUNE_TZ2_BAND, TUNE_TZ2_RECON, TUNE_TZ3 = 0, 1, 2, 3
 
 
 def library_path() -> Path:
@@ -136,6 +137,10 @@ def _declare(L):
         f = getattr(L, name)
         f.restype = _c
         f.argtypes = args + [_e, _c]
+    L.swp_set_tuning.restype = _c
+    L.swp_set_tuning.argtypes = [_i, _i]
+    L.swp_get_tuning.restype = _i
+    L.swp_get_tuning.argtypes = [_i]
     L.swp_nn_conv_wgrad_scratch.restype = _i64
     L.swp_nn_conv_wgrad_scratch.argtypes = [_i, _i, _i, _i, _i, _i]
     L.swp_l2_misfit.restype = ctypes.c_int
@@ -238,6 +243,19 @@ def on_stream(device, fn, *args):
         L.swp_set_stream(ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream))
         rc = fn(*args, err, 1024)
     _raise(rc, err, fn.__name__)
+
+
+def set_tuning(key: int, value: int):
+    """Launch-shape knob (TUNE_*): value > 0 sets it, 0 restores the default."""
+    if load().swp_set_tuning(int(key), int(value)) != 0:
+        raise ValueError(f"bad tuning key {key} or value {value}")
+
+
+def get_tuning(key: int) -> int:
+    v = load().swp_get_tuning(int(key))
+    if v < 0:
+        raise ValueError(f"bad tuning key {key}")
+    return int(v)
 
 
 def _pairs(fn, nt: int, slots: int):

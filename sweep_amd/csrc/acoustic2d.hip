@@ -948,14 +948,12 @@ Ctx make_ctx(const swp_problem &p)
     x.c = make_coef(p, p.lap_coes, p.grad_coes);
     x.pr = Prof{p.pml[0], p.pml[1], p.pml[2], p.pml[3], p.pml[4], p.pml[5]};
     // SWP_KERNELS=percell keeps the one-thread-per-cell kernels (any order);
-    // SWP_TZ sets the z-segment height of the z-marching kernels.
+    // the segment heights of the z-marching kernels come from the tuning
+    // table (swp_set_tuning), else SWP_TZ / SWP_TZ_BAND / SWP_TZ_RECON.
     const char *kern = std::getenv("SWP_KERNELS");
     x.zm = zm_supported(x.g) && !(kern && std::string(kern) == "percell");
-    const char *tz = std::getenv("SWP_TZ");
-    const char *tzr = std::getenv("SWP_TZ_RECON");
-    x.tz_recon = tzr ? std::max(1, std::atoi(tzr)) : 16;
-    const char *tzb = std::getenv("SWP_TZ_BAND");
-    x.plan = make_plan(x.g, tz ? std::max(1, std::atoi(tz)) : 32, tzb ? std::max(1, std::atoi(tzb)) : 8);
+    x.tz_recon = tuning(SWP_TUNE_TZ2_RECON, "SWP_TZ_RECON", 16);
+    x.plan = make_plan(x.g, tuning(SWP_TUNE_TZ2, "SWP_TZ", 32), tuning(SWP_TUNE_TZ2_BAND, "SWP_TZ_BAND", 8));
     return x;
 }
 

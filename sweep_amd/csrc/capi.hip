@@ -3,7 +3,9 @@
 // and answers with a return code plus a message, like the reference's
 // guarded() (csrc/cuda/common/capi.cu:230-242).
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -19,6 +21,7 @@ namespace swp {
 
 namespace {
 thread_local hipStream_t t_stream = nullptr;
+std::atomic<int> g_tuning[SWP_TUNE_COUNT];
 
 struct Entry {
     const char *name;
@@ -88,6 +91,14 @@ void check_problem(const Entry &e, const swp_problem &p)
 }  // namespace
 
 hipStream_t stream() { return t_stream; }
+
+int tuning(int key, const char *env, int def)
+{
+    const int v = (key >= 0 && key < SWP_TUNE_COUNT) ? g_tuning[key].load() : 0;
+    if (v > 0) return v;
+    const char *e = env ? std::getenv(env) : nullptr;
+    return e ? std::max(1, std::atoi(e)) : def;
+}
 
 bool eq_available(int eq) { return eq == 0 || (eq == 1 && a3d::available()) || (eq == 2 && e2d::available()); }
 
@@ -262,6 +273,19 @@ SWP_CAPI int64_t swp_revolve_spine(int32_t nt, int32_t slots, int32_t *pts, int6
     } catch (...) {
         return -1;
     }
+}
+
+SWP_CAPI int swp_set_tuning(int32_t key, int32_t value)
+{
+    if (key < 0 || key >= SWP_TUNE_COUNT || value < 0) return 1;
+    g_tuning[key].store(value);
+    return 0;
+}
+
+SWP_CAPI int32_t swp_get_tuning(int32_t key)
+{
+    if (key < 0 || key >= SWP_TUNE_COUNT) return -1;
+    return g_tuning[key].load();
 }
 
 SWP_CAPI int swp_record_to_canonical(const float *raw, float *canonical, int32_t nfield, int32_t B,

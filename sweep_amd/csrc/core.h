@@ -41,6 +41,10 @@ template <class... A>
 
 hipStream_t stream();
 
+// Launch-shape knob `key` (SWP_TUNE_*): the value set through
+// swp_set_tuning, else the environment variable `env` if set, else `def`.
+int tuning(int key, const char *env, int def);
+
 // Bump allocator over the caller's workspace (or a size-only dry run when
 // base == nullptr), 256-byte aligned slices.
 struct Carver {

@@ -75,3 +75,18 @@ def test_query_sizes_scale_with_the_problem():
     assert ws2 >= 20 * n and tp2 < tp
     with pytest.raises(RuntimeError, match="memory mode"):
         _core.query("acoustic2d_backward", P)
+
+
+def test_tuning_table():
+    for key in (_core.TUNE_TZ2, _core.TUNE_TZ2_BAND, _core.TUNE_TZ2_RECON, _core.TUNE_TZ3):
+        assert _core.get_tuning(key) == 0
+        _core.set_tuning(key, 48)
+        assert _core.get_tuning(key) == 48
+        _core.set_tuning(key, 0)
+        assert _core.get_tuning(key) == 0
+    with pytest.raises(ValueError):
+        _core.set_tuning(7, 1)
+    with pytest.raises(ValueError):
+        _core.set_tuning(_core.TUNE_TZ3, -2)
+    with pytest.raises(ValueError):
+        _core.get_tuning(-1)
