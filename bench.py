@@ -234,8 +234,8 @@ MEASURED_TRAFFIC = {
     ("acoustic2d", 1): 618.1e6,
 
     # profiles/acoustic3d_256x512x512_8shots_bs.md: the eight adj3_kernel region launches of one
-    # step at 4 shots per launch
-    ("acoustic3d", 1): 21487.7e6,
+    # step at 4 shots per launch (64-plane segments)
+    ("acoustic3d", 1): 20652.0e6,
 
     # profiles/elastic2d_512x1024_16shots_ckpt.md: vel_kernel 823.7 MB + stress_kernel 1010.6 MB
     ("elastic2d", 0): 1834.3e6,

@@ -175,8 +175,8 @@ SWP_CAPI int64_t swp_revolve_spine(int32_t nt, int32_t slots, int32_t *pts, int6
 enum {
     SWP_TUNE_TZ2 = 0,        /* 2-D interior z-march segment rows (default 32) */
     SWP_TUNE_TZ2_BAND = 1,   /* 2-D CPML-band segment rows (default 8) */
-    SWP_TUNE_TZ2_RECON = 2,  /* 2-D reconstruction segment rows (default 16) */
-    SWP_TUNE_TZ3 = 3,        /* 3-D z-march segment planes (default 32) */
+    SWP_TUNE_TZ2_RECON = 2,  /* 2-D reconstruction segment rows (default 32) */
+    SWP_TUNE_TZ3 = 3,        /* 3-D z-march segment planes (default 64) */
     SWP_TUNE_COUNT = 4
 };
 SWP_CAPI int swp_set_tuning(int32_t key, int32_t value);

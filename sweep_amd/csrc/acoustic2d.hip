@@ -952,7 +952,7 @@ Ctx make_ctx(const swp_problem &p)
     // table (swp_set_tuning), else SWP_TZ / SWP_TZ_BAND / SWP_TZ_RECON.
     const char *kern = std::getenv("SWP_KERNELS");
     x.zm = zm_supported(x.g) && !(kern && std::string(kern) == "percell");
-    x.tz_recon = tuning(SWP_TUNE_TZ2_RECON, "SWP_TZ_RECON", 16);
+    x.tz_recon = tuning(SWP_TUNE_TZ2_RECON, "SWP_TZ_RECON", 32);
     x.plan = make_plan(x.g, tuning(SWP_TUNE_TZ2, "SWP_TZ", 32), tuning(SWP_TUNE_TZ2_BAND, "SWP_TZ_BAND", 8));
     return x;
 }

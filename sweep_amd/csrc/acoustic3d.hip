@@ -939,7 +939,7 @@ Ctx make_ctx(const swp_problem &p)
         x.pr.b[a] = p.pml[3 * src + 1];
         x.pr.db[a] = p.pml[3 * src + 2];
     }
-    x.tz = tuning(SWP_TUNE_TZ3, "SWP_TZ3", 32);
+    x.tz = tuning(SWP_TUNE_TZ3, "SWP_TZ3", 64);
     const char *kern = std::getenv("SWP_KERNELS");
     x.zm = zm_supported(x.g, x.tz) && !(kern && std::string(kern) == "percell");
     // SWP_SHELL3=percell runs the CPML shell with the per-cell kernels over
