@@ -142,7 +142,12 @@ def _main():
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--iterations", type=int, default=10)
     ap.add_argument("--base", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--batch", type=int, default=2)
+    ap.add_argument("--learned-scales", default="1,0.5,0.25",
+                    help="comma-separated trial scales of the network's update, tried in order")
     args = ap.parse_args()
+    scales = tuple(float(v) for v in args.learned_scales.split(","))
     dev = torch.device("cuda:0")
     shape, dh, dt, f0 = (args.nz, args.nx), 8.0, 8e-4, 8.0
     wav = synthetic.ricker(np.arange(args.nt) * dt - 1.5 / f0, f0)
@@ -152,11 +157,12 @@ def _main():
     batches = [(a, min(a + 32, args.shots)) for a in range(0, args.shots, 32)]
     masks = [torch.tensor(mask[a:b], device=dev) for a, b in batches]
 
-    def make(vt_np, cls=PlainFWI, extra=()):
+    def make(vt_np, cls=PlainFWI, extra=(), **kw):
         vt = torch.tensor(vt_np, device=dev)
         with torch.no_grad():
             obs = [solver(wav, src[a:b], rec[a:b], models=[vt]) for a, b in batches]
-        f = cls(*extra, solver, wav, src, rec, obs, batches, masks, vmin=1400.0, vmax=4600.0, first_step=50.0)
+        f = cls(*extra, solver, wav, src, rec, obs, batches, masks, vmin=1400.0, vmax=4600.0, first_step=50.0,
+                **kw)
         return f, torch.tensor(synthetic.smoothed(vt_np), device=dev)
 
     t0 = time.perf_counter()
@@ -167,14 +173,15 @@ def _main():
           flush=True)
     net = UNet(dev, levels=4, base=args.base, nsrc=3, seed=0)
     t0 = time.perf_counter()
-    losses = train(net, pairs, args.train_steps)
+    losses = train(net, pairs, args.train_steps, batch=args.batch, lr=args.lr)
     torch.cuda.synchronize()
     print(json.dumps({"stage": "train", "steps": args.train_steps, "loss_first": losses[0],
                       "loss_last": float(np.mean(losses[-20:])), "seconds": round(time.perf_counter() - t0, 2)}),
           flush=True)
     held = synthetic.layered_2d(shape, seed=7)
-    for name, cls, extra in (("plain", PlainFWI, ()), ("learned", LearnedFWI, (net,))):
-        f, vp = make(held, cls, extra)
+    for name, cls, extra, kw in (("plain", PlainFWI, (), {}),
+                                 ("learned", LearnedFWI, (net,), {"learned_scales": scales})):
+        f, vp = make(held, cls, extra, **kw)
         vt = torch.tensor(held, device=dev)
         wall = 0.0
         for k in range(args.iterations):
