@@ -51,7 +51,7 @@ WORKLOADS = {
                        nt=3000, order=8, abcn=50, nshots=32, f0=8.0, seed=2, per_launch=32),
     "acoustic3d": dict(name="acoustic3d_256x512x512_nt2000_8shots_bs", shape=(256, 512, 512), dh=10.0,
                        dt=1e-3, nt=2000, order=8, abcn=50, nshots=8, f0=8.0, seed=4, per_launch=4),
-    "elastic2d": dict(name="elastic2d_512x1024_nt4000_16shots_ckpt", shape=(512, 1024), dh=5.0, dt=5e-4,
+    "elastic2d": dict(name="elastic2d_512x1024_nt4000_16shots_bs", shape=(512, 1024), dh=5.0, dt=5e-4,
                       nt=4000, order=8, abcn=50, nshots=16, f0=12.0, seed=6, per_launch=16, nrec=400),
 }
 
@@ -64,11 +64,10 @@ KERNEL_NAMES_E2D = {0: "fwd (velocity + stress step)", 1: "adj (stress/velocity 
 # ----------------------------------------------------------------- problems
 def memory_strategy(spec, elastic):
     """'bs' | 'bs_bf16' | 'ckpt:<chunks>' | 'binomial:<slots>' | 'full' -> an options object
-    (None: the workload's default: checkpointing with 100 chunks for the
-    elastic workload, boundary saving for the acoustic ones)."""
+    (None: the workload's default, boundary saving)."""
     from sweep_amd.options import BoundarySaving, Ckpt, Full
     if spec is None:
-        return Ckpt(chunks=100) if elastic else BoundarySaving()
+        return BoundarySaving()
     kind, _, arg = spec.partition(":")
     if kind == "bs":
         return BoundarySaving()
@@ -362,7 +361,7 @@ def main():
         "config": {"workload": W["name"] if args.nt is None else f"{W['name']}_nt{nt}",
                    "grid": list(W["shape"]), "runtime_grid": list(grid), "nt": nt, "shots": nshots,
                    "shots_per_launch": per_launch, "order": W["order"],
-                   "memory": args.memory or ("ckpt:100" if elastic else "bs"),
+                   "memory": args.memory or "bs",
                    "tape_bytes_per_launch": tape_bytes,
                    **({"autotuned": autotuned} if autotuned else {})},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,

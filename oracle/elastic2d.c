@@ -13,6 +13,10 @@
  *   body-force rho corrections ............ common.cu:40-116
  *   forward / full-storage backward order . csrc/cuda/common/sg_driver.cuh:23-50, 285-340,
  *                                           equations/elastic2d/driver_traits.cuh:380-630
+ *   boundary-saving backward ............. sg_driver.cuh:812: the same reverse steps for
+ *                                           it = nt-1 .. 1 (no step 0); its reconstruction of
+ *                                           the forward state is exact in exact arithmetic, so
+ *                                           the oracle images against the stored velocities
  * Field indices: 0 vx, 1 vz, 2 sxx, 3 szz, 4 sxz.
  */
 #include <math.h>
@@ -361,7 +365,7 @@ int orc_e2d_gradient(const orc_e2d *g, const float *vp, const float *vs, const f
         if (fwd_shot(g, rhob, lam, mu, wb, sl, nsrc, rl, nrec, rec ? rec + (long)b * nrec * nt : NULL, fstride,
                      u_allt)) { rc = -1; break; }
         if (alloc_state(a, n)) { free_state(a); rc = -1; break; }
-        for (int it = nt - 1; it >= 0; --it) {
+        for (int it = nt - 1; it >= (g->bs ? 1 : 0); --it) {
             /* body-force rho correction at the velocity sources, before the residual (common.cu:40-75) */
             for (int k = 0; k < g->nsf; ++k) {
                 const int sf = g->src_fields[k];

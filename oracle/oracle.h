@@ -84,6 +84,7 @@ typedef struct {
     const float *coef;    /* staggered weights c_1..c_M (index 0..M-1), used for M > 4 */
     int nsf, nrf;
     const int *src_fields, *rec_fields;
+    int bs;               /* 1: the boundary-saving backward's reverse loop, it = nt-1 .. 1 */
 } orc_e2d;
 
 /* rec (nrf, B, nrec, nt) */

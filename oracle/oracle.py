@@ -276,6 +276,7 @@ class OrcE2D(ctypes.Structure):
         ("dt", ctypes.c_float), ("dx", ctypes.c_float), ("dz", ctypes.c_float),
         ("az", _F), ("bz", _F), ("azh", _F), ("bzh", _F), ("ax", _F), ("bx", _F), ("axh", _F), ("bxh", _F),
         ("coef", _F), ("nsf", ctypes.c_int), ("nrf", ctypes.c_int), ("src_fields", _I), ("rec_fields", _I),
+        ("bs", ctypes.c_int),
     ]
 
 
@@ -288,7 +289,7 @@ class E2DProblem:
     staggered weights, and the source / receiver field lists."""
 
     def __init__(self, order, shape, dh, dt, nt, abcn=50, max_vel=4500.0, pml_freq=25.0, B=1,
-                 source_fields=("vz",), receiver_fields=("vx", "vz"), variant="ieee"):
+                 source_fields=("vz",), receiver_fields=("vx", "vz"), variant="ieee", bs=False):
         from sweep_amd import pml, scalars
         M = order // 2
         self.M, self.nt, self.B = M, nt, B
@@ -314,6 +315,7 @@ class E2DProblem:
         g.coef = _fp(self.grad)
         g.nsf, g.nrf = len(self.sf), len(self.rf)
         g.src_fields, g.rec_fields = _ip(self.sf), _ip(self.rf)
+        g.bs = 1 if bs else 0
         self.g = g
 
     def pad_model(self, m):

@@ -2,8 +2,8 @@
 // staggered grid with the cpmls CPML (memory variables m = a m + b d), the
 // exact adjoint (stress / velocity prepare + apply), vp / vs / rho imaging
 // with the body-force rho corrections at velocity sources and receivers,
-// full-storage and checkpoint (chunk replay) backwards.  FP32 on the vector
-// ALUs; wave64 rows along x.
+// full-storage, checkpoint (chunk replay, binomial) and boundary-saving
+// backwards.  FP32 on the vector ALUs; wave64 rows along x.
 //
 // Arithmetic follows the reference's elastic2d kernels term for term
 // (csrc/cuda/equations/elastic2d/kernels.cuh:48-330 forward, 547-1055
@@ -14,6 +14,8 @@
 #include "revolve.h"
 
 #include <algorithm>
+#include <cstddef>
+#include <type_traits>
 
 namespace swp {
 namespace e2d {
@@ -84,6 +86,111 @@ __device__ __forceinline__ float sgd(const SCoef &c, const float *__restrict__ u
 __device__ __forceinline__ bool in_pml(const Geo &g, int ix, int iz)
 {
     return ix < g.X0 || ix >= g.X1 || iz < g.Z0 || iz >= g.Z1;
+}
+
+// ---------------------------------------------------------------- boundary strips
+// Boundary saving keeps, per face of the physical box, a band W = M + 1
+// cells wide starting M cells outside the face (boundary/strip.cuh:81-114
+// with offset -M, no tangential pad): top and bottom bands span x in
+// [X0, X1), left and right bands z in [Z0, Z1).  Strip cell s of a shot:
+// top (W, nxp), bottom (W, nxp), left (nzp, W), right (nzp, W).
+__host__ __device__ inline int64_t strip_cells(const Geo &g)
+{
+    return int64_t(2) * (g.M + 1) * ((g.X1 - g.X0) + (g.Z1 - g.Z0));
+}
+
+__device__ __forceinline__ void strip_cell(const Geo &g, int64_t s, int &ix, int &iz)
+{
+    const int W = g.M + 1, nxp = g.X1 - g.X0, nzp = g.Z1 - g.Z0;
+    const int64_t a = int64_t(W) * nxp, c = int64_t(W) * nzp;
+    if (s < a) {
+        iz = g.Z0 - g.M + int(s / nxp);
+        ix = g.X0 + int(s % nxp);
+    } else if (s < 2 * a) {
+        s -= a;
+        iz = g.Z1 - 1 + int(s / nxp);
+        ix = g.X0 + int(s % nxp);
+    } else if (s < 2 * a + c) {
+        s -= 2 * a;
+        iz = g.Z0 + int(s / W);
+        ix = g.X0 - g.M + int(s % W);
+    } else {
+        s -= 2 * a + c;
+        iz = g.Z0 + int(s / W);
+        ix = g.X1 - 1 + int(s % W);
+    }
+}
+
+enum { STRIP_SAVE = 0, STRIP_RESTORE = 1, STRIP_COPY = 2 };
+
+// Fields [f0, f1) at the strip cells: SAVE strip[f, b, s] = src_f, RESTORE
+// dst_f = strip[f, b, s], COPY dst_f = src_f.  strip: one step's (5, B, ns).
+// Grid (ceil(ns / 256), B).
+__global__ void __launch_bounds__(256) strip_kernel(Geo g, Fields src, Fields dst, float *__restrict__ strip, int f0,
+                                                    int f1, int mode)
+{
+    const int64_t ns = strip_cells(g);
+    const int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (s >= ns) return;
+    const int b = blockIdx.y;
+    int ix, iz;
+    strip_cell(g, s, ix, iz);
+    const int64_t i = int64_t(b) * g.n + int64_t(iz) * g.nx + ix;
+    for (int f = f0; f < f1; ++f) {
+        const int64_t k = (int64_t(f) * g.B + b) * ns + s;
+        if (mode == STRIP_SAVE) strip[k] = src.f[f][i];
+        else if (mode == STRIP_RESTORE) dst.f[f][i] = strip[k];
+        else dst.f[f][i] = src.f[f][i];
+    }
+}
+
+// Reverse steps without CPML on [X0 + 1, X1 - 1) x [Z0 + 1, Z1 - 1), the
+// physical box less one cell (abcn + M + 1 from the grid edge), the inverse
+// of the forward updates there (kernels.cuh:365-545): stress first
+// (s -= dt C grad v), then velocity (v -= dt / rho div s), which first copies
+// v into the carriers (cvx, cvz) for the next reverse step's imaging.
+template <int MS>
+__global__ void __launch_bounds__(256) stress_nopml_kernel(Geo g, SCoef c, Fields s, const float *__restrict__ lamb,
+                                                           const float *__restrict__ mub)
+{
+    const int M = MS > 0 ? MS : g.M;
+    const int ix = g.X0 + 1 + int(blockIdx.x) * 64 + int(threadIdx.x);
+    const int iz = g.Z0 + 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
+    const int b = blockIdx.z;
+    if (ix >= g.X1 - 1 || iz >= g.Z1 - 1) return;
+    const int64_t off = int64_t(b) * g.n, i = int64_t(iz) * g.nx + ix, nx = g.nx;
+    const float *vx = s.f[VX] + off, *vz = s.f[VZ] + off;
+    const float dvx_dx = sgd<MS, false>(c, vx, i, 1, g.inv_dx, M);
+    const float dvz_dz = sgd<MS, false>(c, vz, i, nx, g.inv_dz, M);
+    const float dvx_dz = sgd<MS, true>(c, vx, i, nx, g.inv_dz, M);
+    const float dvz_dx = sgd<MS, true>(c, vz, i, 1, g.inv_dx, M);
+    const int64_t mi = int64_t(b) * g.mstride + i;
+    const float la = lamb[mi], mu = mub[mi];
+    s.f[SXX][off + i] -= g.dt * ((la + 2.f * mu) * dvx_dx + la * dvz_dz);
+    s.f[SZZ][off + i] -= g.dt * ((la + 2.f * mu) * dvz_dz + la * dvx_dx);
+    s.f[SXZ][off + i] -= g.dt * mu * (dvx_dz + dvz_dx);
+}
+
+template <int MS>
+__global__ void __launch_bounds__(256) vel_nopml_kernel(Geo g, SCoef c, Fields s, const float *__restrict__ rhob,
+                                                        float *__restrict__ cvx, float *__restrict__ cvz)
+{
+    const int M = MS > 0 ? MS : g.M;
+    const int ix = g.X0 + 1 + int(blockIdx.x) * 64 + int(threadIdx.x);
+    const int iz = g.Z0 + 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
+    const int b = blockIdx.z;
+    if (ix >= g.X1 - 1 || iz >= g.Z1 - 1) return;
+    const int64_t off = int64_t(b) * g.n, i = int64_t(iz) * g.nx + ix, nx = g.nx;
+    const float *sxx = s.f[SXX] + off, *szz = s.f[SZZ] + off, *sxz = s.f[SXZ] + off;
+    const float dsxx_dx = sgd<MS, true>(c, sxx, i, 1, g.inv_dx, M);
+    const float dsxz_dz = sgd<MS, false>(c, sxz, i, nx, g.inv_dz, M);
+    const float dsxz_dx = sgd<MS, false>(c, sxz, i, 1, g.inv_dx, M);
+    const float dszz_dz = sgd<MS, true>(c, szz, i, nx, g.inv_dz, M);
+    const float inv_rho = 1.f / rhob[int64_t(b) * g.mstride + i];
+    cvx[off + i] = s.f[VX][off + i];
+    cvz[off + i] = s.f[VZ][off + i];
+    s.f[VX][off + i] -= g.dt * inv_rho * (dsxx_dx + dsxz_dz);
+    s.f[VZ][off + i] -= g.dt * inv_rho * (dsxz_dx + dszz_dz);
 }
 
 // ---------------------------------------------------------------- models
@@ -430,7 +537,8 @@ struct Work {
     Fields a;      // adjoint state
     Fields qp;     // q (0..3), p (4..7)
     float *zero;   // (2, B, n) zero velocities: v(nt)
-    float *carry;  // (2, B, n) v(chunk end) for the checkpoint reverse; v(it + 1) for the binomial one
+    float *carry;  // (2, B, n) v(chunk end) for the checkpoint reverse; v(it + 1) for the binomial and
+                   // boundary-saving ones
     float *chunk;  // (I, 2, B, n) replayed velocities; (2, B, n) v(it) for the binomial reverse
 };
 
@@ -452,6 +560,8 @@ void carve(Carver &cv, const swp_problem &p, const Geo &g, int kind, Work &w)
     } else if (p.memory == SWP_MEM_BINOMIAL) {
         w.carry = cv.take<float>(2 * N);
         w.chunk = cv.take<float>(2 * N);
+    } else if (p.memory == SWP_MEM_BOUNDARY) {
+        w.carry = cv.take<float>(2 * N);
     }
 }
 
@@ -460,6 +570,8 @@ constexpr int kSnap = NSTATE;   // every field of the state
 struct Tape {
     float *vel = nullptr;    // FULL: (nt, 2, B, n) velocities after each step's velocity update
     float *snaps = nullptr;  // CKPT: (nchunk, 13, B, n)
+    float *strips = nullptr; // BOUNDARY: (nt, 5, B, ns) the five fields' strips after each step
+    float *last = nullptr;   // BOUNDARY: (5, B, n) the five fields after the last step
     int nchunk = 0;
 };
 
@@ -477,8 +589,21 @@ Tape carve_tape(Carver &cv, const swp_problem &p, const Geo &g)
         SWP_CHECK(p.ckpt_interval >= 1, "binomial checkpointing needs >= 1 snapshot slot");
         t.nchunk = p.ckpt_interval;
         t.snaps = cv.take<float>(int64_t(t.nchunk) * kSnap * N);
+    } else if (p.memory == SWP_MEM_BOUNDARY) {
+        t.strips = cv.take<float>(int64_t(g.nt) * 5 * g.B * strip_cells(g));
+        t.last = cv.take<float>(5 * N);
     }
     return t;
+}
+
+// The strip pass over fields [f0, f1) of one step's (5, B, ns) slice.
+void strips(const Geo &g, const Fields &src, const Fields &dst, float *slice, int f0, int f1, int mode)
+{
+    const int64_t ns = strip_cells(g);
+    if (ns <= 0) return;
+    hipLaunchKernelGGL(strip_kernel, dim3(unsigned((ns + 255) / 256), unsigned(g.B)), dim3(256), 0, stream(), g, src,
+                       dst, slice, f0, f1, mode);
+    SWP_LAUNCHED();
 }
 
 void derive_models(const swp_problem &p, const Geo &g, const Models &md)
@@ -558,14 +683,24 @@ void restore(const Geo &g, Fields &s, const float *src)
     for (int k = 0; k < kSnap; ++k) copy(s.f[k], src + k * N, N);
 }
 
+// The velocities one reverse step images against: v(it) and v(it + 1), each
+// as (vx, vz) fields of (B, n).
+struct VelPair {
+    const float *vxn, *vzn, *vxp, *vzp;
+};
+
+VelPair pair_of(const float *vnow, const float *vnext, int64_t N) { return {vnow, vnow + N, vnext, vnext + N}; }
+
 // One reverse step it: rho correction at the sources, signed residuals,
 // imaging against v(it) / v(it+1), the adjoint step (it > 0), rho correction
-// at the receivers (sg_driver.cuh:40-50).
-void reverse_step(const Ctx &x, const swp_problem &p, const swp_backward_io &io, Work &w, int it, const float *vnow,
-                  const float *vnext)
+// at the receivers (sg_driver.cuh:40-50).  With `between` the boundary-saving
+// reconstruction of the stresses runs after the residuals and before the
+// imaging (driver_traits.cuh:690-733).
+template <class Between = std::nullptr_t>
+void reverse_step(const Ctx &x, const swp_problem &p, const swp_backward_io &io, Work &w, int it, const VelPair &v,
+                  Between between = nullptr)
 {
     const Geo &g = x.g;
-    const int64_t N = int64_t(g.B) * g.n;
     float *gvp = io.grad_models[0], *gvs = io.grad_models[1], *grho = io.grad_models[2];
     SWP_CHECK(gvp && gvs && grho, "elastic2d backward: grad_models[0..2] (vp, vs, rho) are required");
     {
@@ -587,7 +722,8 @@ void reverse_step(const Ctx &x, const swp_problem &p, const swp_backward_io &io,
         }
         SWP_LAUNCHED();
     }
-    ImgArgs m{vnow, vnow + N, vnext, vnext + N, p.models[0], p.models[1], p.models[2], gvp, gvs, grho};
+    if constexpr (!std::is_same_v<Between, std::nullptr_t>) between();
+    ImgArgs m{v.vxn, v.vzn, v.vxp, v.vzp, p.models[0], p.models[1], p.models[2], gvp, gvs, grho};
     {
         TimedScope ts(K_RECON);
         by_order(g.M, [&](auto ms) {
@@ -611,9 +747,9 @@ void reverse_step(const Ctx &x, const swp_problem &p, const swp_backward_io &io,
     TimedScope ts(K_OTHER);
     for (int k = 0; k < x.nrf; ++k) {
         if (x.rf[k] > 1 || p.nrec == 0) continue;
-        const int64_t fo = x.rf[k] == 0 ? 0 : N;
+        const float *fn = x.rf[k] == 0 ? v.vxn : v.vzn, *fp = x.rf[k] == 0 ? v.vxp : v.vzp;
         hipLaunchKernelGGL(rho_rec_kernel, dim3(unsigned((p.nrec + 255) / 256), unsigned(g.B)), dim3(256), 0, stream(),
-                           g, grho, vnow + fo, vnext + fo, p.models[2],
+                           g, grho, fn, fp, p.models[2],
                            io.adjoint_source + int64_t(k) * g.B * p.nrec * g.nt, p.receivers_loc, p.nrec, it);
     }
     SWP_LAUNCHED();
@@ -632,8 +768,8 @@ int64_t workspace_bytes(const swp_problem &p, int kind)
 
 int64_t tape_bytes(const swp_problem &p)
 {
-    SWP_CHECK(p.memory != SWP_MEM_BOUNDARY,
-              "elastic2d: boundary saving is not part of this core yet; use Full() or Ckpt() memory");
+    SWP_CHECK(!(p.memory == SWP_MEM_BOUNDARY && (p.flags & SWP_FLAG_BF16_STRIPS)),
+              "elastic2d: boundary saving keeps float32 strips (bf16 strips are acoustic only)");
     const Geo g = make_geo(p);
     Carver cv(nullptr, 0);
     carve_tape(cv, p, g);
@@ -661,8 +797,71 @@ void forward(const swp_problem &p, float *record)
         for (; sp < spine.size() && spine[sp].step == it; ++sp)
             snapshot(g, w.s, t.snaps + int64_t(spine[sp].slot) * kSnap * N);
         forward_step(x, p, w, it, t.vel ? t.vel + int64_t(it) * 2 * N : nullptr, record);
+        // the five fields' strips after the step's injection (sg_driver.cuh:324-326)
+        if (t.strips) strips(g, w.s, w.s, t.strips + int64_t(it) * 5 * g.B * strip_cells(g), 0, 5, STRIP_SAVE);
+    }
+    if (t.last)
+        for (int f = 0; f < 5; ++f) copy(t.last + f * N, w.s.f[f], N);
+}
+
+namespace {
+
+// Boundary-saving reverse (sg_driver.cuh:740-812, elastic2d/driver_traits.cuh:
+// 634-826): the forward state is rebuilt backwards from the last state, the
+// reverse no-CPML steps inside the box and the saved strips at its rim; the
+// reverse loop runs it = nt - 1 down to 1 (no step 0).  At step it the
+// reconstruction holds the state after forward step it; with the step's
+// source removed its velocities are v(it); the stresses are stepped back and
+// their strips restored (from forward step it - 1) before the imaging, which
+// reads v(it) and the carriers v(it + 1); after the adjoint step the
+// velocities are stepped back, leaving v(it) in the carriers.
+void backward_bs(const Ctx &x, const swp_problem &p, const swp_backward_io &io, Work &w, const Tape &t)
+{
+    const Geo &g = x.g;
+    const int64_t N = int64_t(g.B) * g.n;
+    const int64_t slice = int64_t(5) * g.B * strip_cells(g);
+    for (int f = 0; f < 5; ++f) copy(w.s.f[f], t.last + f * N, N);
+    fill_zero(w.carry, 2 * N);   // v(nt) = 0
+    Fields car{};
+    car.f[VX] = w.carry;
+    car.f[VZ] = w.carry + N;
+    const dim3 box = grid_for(g.X1 - g.X0 - 2, g.Z1 - g.Z0 - 2, g.B);
+    for (int it = g.nt - 1; it >= 1; --it) {
+        float *sl = t.strips + int64_t(it - 1) * slice;
+        auto recon_stress = [&] {
+            TimedScope ts(K_OTHER);
+            for (int k = 0; k < x.nsf; ++k) {   // remove step it's source (sign -1)
+                PointArgs a = point_args(g);
+                a.it = it;
+                a.u_add = w.s.f[x.sf[k]];
+                a.add_w = p.source;
+                a.add_loc = p.sources_loc;
+                a.nadd = p.nsrc;
+                a.sign = -1.f;
+                launch_points(a, g.B);
+            }
+            if (g.X1 - g.X0 > 2 && g.Z1 - g.Z0 > 2)
+                by_order(g.M, [&](auto ms) {
+                    hipLaunchKernelGGL(stress_nopml_kernel<decltype(ms)::value>, box, dim3(64, 4), 0, stream(), g,
+                                       x.c, w.s, w.md.lam, w.md.mu);
+                });
+            SWP_LAUNCHED();
+            strips(g, w.s, w.s, sl, SXX, SXZ + 1, STRIP_RESTORE);
+        };
+        reverse_step(x, p, io, w, it, VelPair{w.s.f[VX], w.s.f[VZ], car.f[VX], car.f[VZ]}, recon_stress);
+        TimedScope ts(K_OTHER);
+        strips(g, w.s, car, sl, VX, VZ + 1, STRIP_COPY);   // v(it) at the rim into the carriers
+        if (g.X1 - g.X0 > 2 && g.Z1 - g.Z0 > 2)
+            by_order(g.M, [&](auto ms) {
+                hipLaunchKernelGGL(vel_nopml_kernel<decltype(ms)::value>, box, dim3(64, 4), 0, stream(), g, x.c, w.s,
+                                   p.models[2], car.f[VX], car.f[VZ]);
+            });
+        SWP_LAUNCHED();
+        strips(g, w.s, w.s, sl, VX, VZ + 1, STRIP_RESTORE);
     }
 }
+
+}  // namespace
 
 void backward(const swp_problem &p, const swp_backward_io &io)
 {
@@ -683,8 +882,12 @@ void backward(const swp_problem &p, const swp_backward_io &io)
         for (int it = g.nt - 1; it >= 0; --it) {
             const float *vnow = t.vel + int64_t(it) * 2 * N;
             const float *vnext = it + 1 < g.nt ? t.vel + int64_t(it + 1) * 2 * N : w.zero;
-            reverse_step(x, p, io, w, it, vnow, vnext);
+            reverse_step(x, p, io, w, it, pair_of(vnow, vnext, N));
         }
+        return;
+    }
+    if (p.memory == SWP_MEM_BOUNDARY) {
+        backward_bs(x, p, io, w, t);
         return;
     }
     if (p.memory == SWP_MEM_BINOMIAL) {
@@ -699,7 +902,7 @@ void backward(const swp_problem &p, const swp_backward_io &io)
             case REV_LOAD: restore(g, w.s, t.snaps + int64_t(o.arg) * kSnap * N); break;
             default:
                 forward_step(x, p, w, o.arg, w.chunk, nullptr);
-                reverse_step(x, p, io, w, o.arg, w.chunk, w.carry);
+                reverse_step(x, p, io, w, o.arg, pair_of(w.chunk, w.carry, N));
                 std::swap(w.chunk, w.carry);
                 break;
             }
@@ -716,7 +919,7 @@ void backward(const swp_problem &p, const swp_backward_io &io)
         for (int it = it1 - 1; it >= it0; --it) {
             const float *vnow = w.chunk + int64_t(it - it0) * 2 * N;
             const float *vnext = it + 1 < it1 ? w.chunk + int64_t(it + 1 - it0) * 2 * N : w.carry;
-            reverse_step(x, p, io, w, it, vnow, vnext);
+            reverse_step(x, p, io, w, it, pair_of(vnow, vnext, N));
         }
         copy(w.carry, w.chunk, 2 * N);   // v(it0) is v(it1) of the next older chunk
     }

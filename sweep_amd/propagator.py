@@ -172,14 +172,12 @@ class PropTorch(torch.nn.Module):
         if memory is not None:
             memory = as_memory_strategy(memory)
         else:
-            # the elastic core has no boundary-saving backward yet: checkpointing
-            # gives the same gradient as full storage at bounded memory
-            memory = Ckpt() if isinstance(equation, Elastic) else BoundarySaving()
+            memory = BoundarySaving()   # the compiled path's default (propagator/torch.py:594-599)
         if isinstance(memory, BoundarySaving):
-            if isinstance(equation, Elastic):
-                raise NotImplementedError("Elastic with BoundarySaving is not part of this build yet; "
-                                          "use memory=Ckpt() (default) or Full().")
             memory.check_supported()
+            if isinstance(equation, Elastic) and memory.storage_dtype != "fp32":
+                raise NotImplementedError("Elastic boundary saving keeps float32 strips in this build "
+                                          "(BoundarySaving(storage_dtype='fp32')).")
         if isinstance(memory, Ckpt) and memory.storage != "gpu":
             raise NotImplementedError("Ckpt(storage='cpu') is not part of this build.")
         self.__dict__["_built"] = False

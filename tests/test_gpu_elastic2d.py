@@ -1,7 +1,11 @@
 """GPU parity of the 2-D elastic path: libswp_core through PropTorch against
 the reference core's fixtures (golden/refcore_e2d_*.npz) and the oracle.
 
-Bars: records rel-L2 <= 1e-5, vp / vs / rho gradients rel-L2 <= 1e-4.
+Bars: records rel-L2 <= 1e-5, vp / vs / rho gradients rel-L2 <= 1e-4.  The
+boundary-saving backward (reverse loop it = nt-1 .. 1, sg_driver.cuh:812) is
+compared with the oracle's boundary-saving loop, which images against the
+stored forward velocities: the GPU rebuilds them by reverse steps and saved
+strips, so the difference is that reconstruction's float32 rounding.
 """
 from __future__ import annotations
 
@@ -23,10 +27,10 @@ def _dev():
 
 
 def _memory(name):
-    from sweep_amd.options import Ckpt, Full
+    from sweep_amd.options import BoundarySaving, Ckpt, Full
     if name.startswith("binomial"):   # "binomial<k>": Ckpt(mode='recursive', count=k)
         return Ckpt(mode="recursive", count=int(name[len("binomial"):]))
-    return {"full": Full(), "ckpt": Ckpt(chunks=7)}[name]
+    return {"full": Full(), "ckpt": Ckpt(chunks=7), "bs": BoundarySaving()}[name]
 
 
 def _prop(order, shape, dh, dt, abcn, mode, dev, src_type=None, rec_type=None):
@@ -88,10 +92,10 @@ def _random_case(order, seed, shape=(30, 40), abcn=8, nt=240):
                 src=src, rec=rec, rng=rng)
 
 
-def _oracle_run(c, src_type, rec_type, res):
+def _oracle_run(c, src_type, rec_type, res, bs=False):
     from oracle.oracle import E2DProblem
     P = E2DProblem(c["order"], c["shape"], c["dh"], c["dt"], c["nt"], abcn=c["abcn"], B=c["B"],
-                   source_fields=src_type, receiver_fields=rec_type)
+                   source_fields=src_type, receiver_fields=rec_type, bs=bs)
     wav = np.broadcast_to(c["wav"], (c["B"], 1, c["nt"])).copy()
     o = P.gradient(*[P.pad_model(m) for m in c["models"]], wav, P.shift(c["src"][:, None, :]), P.shift(c["rec"]),
                    res)
@@ -99,7 +103,7 @@ def _oracle_run(c, src_type, rec_type, res):
 
 
 @pytest.mark.parametrize("order", [2, 4, 6, 8, 10])
-@pytest.mark.parametrize("mode", ["full", "ckpt"])
+@pytest.mark.parametrize("mode", ["full", "ckpt", "bs"])
 @pytest.mark.parametrize("fields", [(("vz",), ("vx", "vz")), (("vx",), ("vz", "sxx")), (("sxx",), ("szz",))],
                          ids=["vz-vxvz", "vx-vzsxx", "sxx-szz"])
 def test_matches_oracle(order, mode, fields):
@@ -110,7 +114,7 @@ def test_matches_oracle(order, mode, fields):
     res = c["rng"].standard_normal((len(rec_type), c["B"], c["rec"].shape[1], c["nt"])).astype(np.float32)
     got = _gpu_run(order, c["shape"], c["dh"], c["dt"], c["abcn"], mode, c["models"], c["wav"], c["src"], c["rec"],
                    res, dev, list(src_type), list(rec_type))
-    ref = _oracle_run(c, src_type, rec_type, res)
+    ref = _oracle_run(c, src_type, rec_type, res, bs=mode == "bs")
     errs = {k: rel_l2(got[k], ref[k]) for k in ("rec", "grad_vp", "grad_vs", "grad_rho")}
     print(f"[elastic2d] {order} {mode} {fields} {errs}", flush=True)
     assert errs["rec"] <= 1e-5
@@ -141,7 +145,7 @@ def test_refusals():
     from sweep_amd.propagator import PropTorch
     with pytest.raises(NotImplementedError):
         PropTorch(Elastic(spatial_order=8, device=dev), shape=(30, 40), dh=5.0, dt=5e-4, device=dev,
-                  memory=BoundarySaving())
+                  memory=BoundarySaving(storage_dtype="bf16"))
     c = _random_case(4, seed=3)
     solver = _prop(4, c["shape"], c["dh"], c["dt"], c["abcn"], "ckpt", dev)
     w = torch.tensor(c["wav"], device=dev, requires_grad=True)
