@@ -148,7 +148,7 @@ int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *
 int run_elastic(const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zx,
                 const float *const *pml, const float *vp, const float *vs, const float *rho, const float *grad,
                 const float *src, const int *sloc, int nsrc, const int *rloc, int nrec, const float *res,
-                float *rec_out, float *g_vp, float *g_vs, float *g_rho, char *err, int cap)
+                float *rec_out, float *g_vp, float *g_vs, float *g_rho, char *err, int cap, int bs = 0)
 {
     try {
         const int nz = grid[0], nx = grid[1];
@@ -168,9 +168,21 @@ int run_elastic(const int *grid, int B, int M, int nt, int abcn, float dt, const
         std::vector<float> spacing{spacing_zx[1], spacing_zx[0]};
         const int sfield = 1;
         const int rfields[2] = {0, 1};
-        std::vector<float> u_allt(size_t(nt) * 2 * B * n, 0.f);
+        std::vector<float> u_allt(bs ? 0 : size_t(nt) * 2 * B * n, 0.f);
         const std::vector<int64_t> ashape{nt, 2, B, nz, nx};
         std::vector<float> lap_dummy(size_t(M + 1), 0.f);
+        // boundary-saving storage: top / bottom (5, nt, B, W, nx_phys), left /
+        // right (5, nt, B, nz_phys, W), last state (5, 2, B, 1, nz, nx)
+        const int W = M + 1, nxp = nx - 2 * (abcn + M), nzp = nz - 2 * (abcn + M);
+        const int64_t tb = int64_t(5) * nt * B * W * nxp, lr = int64_t(5) * nt * B * nzp * W;
+        std::vector<float> s_top(bs ? tb : 0), s_bot(bs ? tb : 0), s_left(bs ? lr : 0), s_right(bs ? lr : 0);
+        std::vector<float> last(bs ? size_t(5) * 2 * B * n : 0, 0.f);
+        std::vector<Buf> bgpu;
+        if (bs) {
+            bgpu = {mk(s_top.data(), {5, nt, B, W, nxp}), mk(s_bot.data(), {5, nt, B, W, nxp}),
+                    mk(s_left.data(), {5, nt, B, nzp, W}), mk(s_right.data(), {5, nt, B, nzp, W})};
+        }
+        const Buf last_b = mk(last.data(), {5, 2, B, 1, nz, nx});
 
         ForwardInputCore f{};
         f.models = list(models);
@@ -186,20 +198,34 @@ int run_elastic(const int *grid, int B, int M, int nt, int abcn, float dt, const
         f.receiver_field_indices = IntSpan{rfields, 2};
         f.pml_vals = list(pml_b);
         f.wavefields = list(fw);
-        f.save_all_wavefields = res != nullptr;
+        f.save_all_wavefields = res != nullptr && !bs;
+        if (bs && res) {
+            f.use_boundary_saving = true;
+            f.boundary_gpu = list(bgpu);
+            f.last_two = last_b;
+        }
         f.nt = unsigned(nt);
         f.dt = dt;
         f.spacing = FloatSpan{spacing.data(), 2};
         f.record_out = mk(rec_out, {2, B, nrec, nt});
-        if (res) f.u_allt_out = mk(u_allt.data(), ashape);
+        if (res && !bs) f.u_allt_out = mk(u_allt.data(), ashape);
         elastic2d::forward_core(f);
         if (!res) return 0;
 
         std::vector<Buf> grads{mk(g_vp, gshape), mk(g_vs, gshape), mk(g_rho, gshape)};
         BackwardInputCore bi{};
-        bi.u_forward = mk(u_allt.data(), ashape);
+        std::vector<std::vector<float>> recon(bs ? 7 : 0, std::vector<float>(size_t(B * n), 0.f));
+        std::vector<Buf> rw;
+        for (auto &r : recon) rw.push_back(mk(r.data(), gshape));
+        if (bs) {
+            bi.u_last_two = last_b;
+            bi.boundary_gpu = list(bgpu);
+            bi.forward_wavefields = list(rw);
+        } else {
+            bi.u_forward = mk(u_allt.data(), ashape);
+        }
         bi.adjoint_wavefields = list(aw);
-        bi.adjoint_workspace = list(ws);
+        bi.adjoint_workspace = BufList{ws.data(), int64_t(bs ? 8 : ws.size())};
         bi.derived_models = list(dm);
         bi.models = list(models);
         bi.adjoint_source = mk(res, {2, B, nrec, nt});
@@ -218,7 +244,8 @@ int run_elastic(const int *grid, int B, int M, int nt, int abcn, float dt, const
         bi.spacing = f.spacing;
         bi.compute_illumination = false;
         bi.grads_out = list(grads);
-        elastic2d::backward_core(bi);
+        if (bs) elastic2d::backward_bs_core(bi);
+        else elastic2d::backward_core(bi);
         return 0;
     } catch (const std::exception &e) {
         put_err(e.what(), err, cap);
@@ -240,6 +267,17 @@ __attribute__((visibility("default"))) int ref_elastic_run(
 {
     return run_elastic(grid, B, M, nt, abcn, dt, spacing_zx, pml, vp, vs, rho, grad, src, sloc, nsrc, rloc, nrec,
                        res, rec_out, g_vp, g_vs, g_rho, err, cap);
+}
+
+// The same with boundary saving (backward_bs_core).
+__attribute__((visibility("default"))) int ref_elastic_run_bs(
+    const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zx, const float *const *pml,
+    const float *vp, const float *vs, const float *rho, const float *grad, const float *src, const int *sloc,
+    int nsrc, const int *rloc, int nrec, const float *res, float *rec_out, float *g_vp, float *g_vs, float *g_rho,
+    char *err, int cap)
+{
+    return run_elastic(grid, B, M, nt, abcn, dt, spacing_zx, pml, vp, vs, rho, grad, src, sloc, nsrc, rloc, nrec,
+                       res, rec_out, g_vp, g_vs, g_rho, err, cap, 1);
 }
 
 __attribute__((visibility("default"))) int ref_acoustic_run(

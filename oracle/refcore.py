@@ -32,6 +32,7 @@ def lib():
         L = ctypes.CDLL(str(LIB))
         L.ref_acoustic_run.restype = ctypes.c_int
         L.ref_elastic_run.restype = ctypes.c_int
+        L.ref_elastic_run_bs.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -71,7 +72,7 @@ def run(P, vp_pad, wav, sloc, rloc, res=None):
     return dict(rec=rec, grad_vp=gv, grad_wavelet=gw)
 
 
-def run_elastic(shape_rt, M, nt, abcn, dt, dh_zx, prof, vp, vs, rho, grad, src, sloc, rloc, res=None):
+def run_elastic(shape_rt, M, nt, abcn, dt, dh_zx, prof, vp, vs, rho, grad, src, sloc, rloc, res=None, bs=False):
     """Forward (records vx, vz) and, with ``res`` (2, B, nrec, nt), the
     full-storage backward of the reference's elastic 2-D core.  Arrays on
     the runtime grid (B, nz, nx); ``prof`` = [az, bz, azh, bzh, ax, bx, axh,
@@ -89,7 +90,8 @@ def run_elastic(shape_rt, M, nt, abcn, dt, dh_zx, prof, vp, vs, rho, grad, src, 
     g = [np.zeros(vp.shape, np.float32) for _ in range(3)]
     sp = np.array(dh_zx, dtype=np.float32)
     err = ctypes.create_string_buffer(1024)
-    rc = lib().ref_elastic_run(grid.ctypes.data_as(_I), B, M, nt, abcn, ctypes.c_float(dt), _fp(sp), parr,
+    fn = lib().ref_elastic_run_bs if bs else lib().ref_elastic_run
+    rc = fn(grid.ctypes.data_as(_I), B, M, nt, abcn, ctypes.c_float(dt), _fp(sp), parr,
                                _fp(vp), _fp(vs), _fp(rho), _fp(grad), _fp(src), sloc.ctypes.data_as(_I), nsrc,
                                rloc.ctypes.data_as(_I), nrec, _fp(res), _fp(rec), _fp(g[0]), _fp(g[1]), _fp(g[2]),
                                err, 1024)

@@ -3,13 +3,16 @@
 Run in the build container only, after ``oracle/build_ref.sh`` (the reference
 tree is not present on GPU machines)::
 
-    cd /tmp && python /root/repo/tests/golden/gen_refcore.py
+    cd /tmp && python /root/repo/tests/golden/gen_refcore.py           # every fixture
+    cd /tmp && python /root/repo/tests/golden/gen_refcore.py e2d_bs    # the elastic boundary-saving ones
 
 The reference's CPU host core (its own CUDA sources compiled as C++, see
 ``oracle/build_ref.sh``) runs each case in full-storage mode through
 ``oracle/ref_harness.cpp``: the forward record, then the adjoint driven by a
 seeded residual, giving the model gradient on the runtime grid and the
-wavelet gradient.  Problem assembly (padding, coordinate shift, CPML
+wavelet gradient.  The elastic cases are also run with boundary saving
+(``refcore_e2d_bs_*``: the reference's backward_bs_core rebuilding the
+forward state from its saved strips).  Problem assembly (padding, coordinate shift, CPML
 profiles, FD tables) is the oracle's, whose profiles and tables the table
 tests pin bit-exactly against the reference's own builders.  Only inputs and
 outputs are stored, as ``.npz`` arrays.
@@ -59,7 +62,7 @@ CASES_3D = [
 ]
 
 
-def main():
+def main(only=None):
     if os.getcwd().startswith("/root/reference"):
         raise SystemExit("run from outside /root/reference")
     sys.path.insert(0, str(ROOT))
@@ -67,7 +70,7 @@ def main():
     if not refcore.available():
         raise SystemExit("oracle/_ref is not built: run oracle/build_ref.sh first")
     dh, dt, abcn, f0 = 10.0, 1e-3, 10, 15.0
-    for ndim, cases in ((2, CASES_2D), (3, CASES_3D)):
+    for ndim, cases in (() if only == "e2d_bs" else ((2, CASES_2D), (3, CASES_3D))):
         for name, order, shape, nsrc, B, nt in cases:
             rng = np.random.default_rng(1000 * ndim + order)
             ab = abcn if ndim == 2 else 8
@@ -100,6 +103,7 @@ def main():
                                 rec=out["rec"], grad_vp=out["grad_vp"], grad_wavelet=out["grad_wavelet"])
             print(path.name, os.path.getsize(path))
     # elastic 2-D: vz sources, vx + vz receivers, layered vp with vs = vp / 1.8, Gardner rho
+    modes = (True,) if only == "e2d_bs" else (False, True)
     dh, dt, abcn, f0 = 5.0, 5e-4, 10, 15.0
     for name, order, shape, B, nt in CASES_E2D:
         rng = np.random.default_rng(3000 + order)
@@ -116,15 +120,17 @@ def main():
         rec[..., 1] = 2
         wavB = np.broadcast_to(wav, (B, 1, nt)).copy()
         res = rng.standard_normal((2, B, nrec, nt)).astype(np.float32)
-        out = refcore.run_elastic((P.nz, P.nx), P.M, nt, abcn, dt, (dh, dh), P.prof, P.pad_model(vp),
-                                  P.pad_model(vs), P.pad_model(rho), P.grad, wavB, P.shift(src), P.shift(rec), res)
-        path = OUT / f"refcore_e2d_{name}.npz"
-        np.savez_compressed(path, order=order, shape=np.array(shape), dh=dh, dt=dt, nt=nt, abcn=abcn, B=B,
-                            vp=vp, vs=vs, rho=rho, wavelet=wav, sources=src, receivers=rec, residual=res,
-                            rec=out["rec"], grad_vp=P.crop(out["grad_vp"]), grad_vs=P.crop(out["grad_vs"]),
-                            grad_rho=P.crop(out["grad_rho"]))
-        print(path.name, os.path.getsize(path))
+        for bs in modes:
+            out = refcore.run_elastic((P.nz, P.nx), P.M, nt, abcn, dt, (dh, dh), P.prof, P.pad_model(vp),
+                                      P.pad_model(vs), P.pad_model(rho), P.grad, wavB, P.shift(src), P.shift(rec),
+                                      res, bs=bs)
+            path = OUT / (f"refcore_e2d_bs_{name}.npz" if bs else f"refcore_e2d_{name}.npz")
+            np.savez_compressed(path, order=order, shape=np.array(shape), dh=dh, dt=dt, nt=nt, abcn=abcn, B=B,
+                                vp=vp, vs=vs, rho=rho, wavelet=wav, sources=src, receivers=rec, residual=res,
+                                rec=out["rec"], grad_vp=P.crop(out["grad_vp"]), grad_vs=P.crop(out["grad_vs"]),
+                                grad_rho=P.crop(out["grad_rho"]))
+            print(path.name, os.path.getsize(path))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

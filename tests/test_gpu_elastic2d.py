@@ -18,6 +18,7 @@ from conftest import GOLDEN, rel_l2
 pytestmark = pytest.mark.gpu
 
 FIXTURES = ["refcore_e2d_o2", "refcore_e2d_o4", "refcore_e2d_o8"]
+BS_FIXTURES = ["refcore_e2d_bs_o2", "refcore_e2d_bs_o4", "refcore_e2d_bs_o8"]
 
 
 def _dev():
@@ -67,6 +68,26 @@ def test_matches_reference_core_fixture(case, mode):
     for k in ("grad_vp", "grad_vs", "grad_rho"):
         errs[k] = rel_l2(got[k], z[k].sum(0))
     print(f"[elastic2d] {case} {mode} {errs}", flush=True)
+    assert errs["rec"] <= 1e-5
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        assert errs[k] <= 1e-4, k
+
+
+@pytest.mark.parametrize("case", BS_FIXTURES)
+def test_boundary_saving_matches_reference_core_fixture(case):
+    """The GPU's boundary-saving backward against the reference core's own
+    (backward_bs_core): both rebuild the forward state, each with its own
+    float32 rounding."""
+    dev = _dev()
+    print(f"[elastic2d] fixture={case} mode=bs", flush=True)
+    z = np.load(GOLDEN / f"{case}.npz")
+    got = _gpu_run(int(z["order"]), z["shape"], float(z["dh"]), float(z["dt"]), int(z["abcn"]), "bs",
+                   [z["vp"], z["vs"], z["rho"]], z["wavelet"], z["sources"][:, 0, :], z["receivers"], z["residual"],
+                   dev)
+    errs = {"rec": rel_l2(got["rec"], z["rec"])}
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        errs[k] = rel_l2(got[k], z[k].sum(0))
+    print(f"[elastic2d] {case} bs {errs}", flush=True)
     assert errs["rec"] <= 1e-5
     for k in ("grad_vp", "grad_vs", "grad_rho"):
         assert errs[k] <= 1e-4, k
