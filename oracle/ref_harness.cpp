@@ -55,7 +55,7 @@ template <int NDIM>
 int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zyx,
         const float *const *pml, const float *vp, const float *lap, const float *grad, const float *src,
         const int *sloc, int nsrc, const int *rloc, int nrec, const float *res, float *rec_out,
-        float *grad_vp, float *grad_w, char *err, int cap)
+        float *grad_vp, float *grad_w, char *err, int cap, int bs = 0)
 {
     try {
         const int nz = grid[0], ny = NDIM == 3 ? grid[1] : 1, nx = grid[NDIM - 1];
@@ -99,20 +99,47 @@ int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *
         f.receiver_field_indices = IntSpan{&fidx, 1};
         f.pml_vals = list(pml_b);
         f.wavefields = list(fw);
-        f.save_all_wavefields = res != nullptr;
-        f.use_boundary_saving = false;
+        // boundary saving (2-D): the saver stores u only, strips M + 1 wide at
+        // offset 0, top / bottom (1, nt, B, W, nx_phys), left / right
+        // (1, nt, B, nz_phys, W); last state (1, 2, B, 1, nz, nx)
+        const int W = M + 1, nxp = nx - 2 * (abcn + M), nzp = nz - 2 * (abcn + M);
+        const bool use_bs = bs && NDIM == 2 && res;
+        std::vector<float> s_tb(use_bs ? size_t(2) * nt * B * W * nxp : 0), s_lr(use_bs ? size_t(2) * nt * B * nzp * W : 0);
+        std::vector<float> last(use_bs ? size_t(2) * B * n : 0, 0.f);
+        std::vector<Buf> bgpu;
+        if (use_bs) {
+            const int64_t tb = int64_t(nt) * B * W * nxp, lr = int64_t(nt) * B * nzp * W;
+            bgpu = {mk(s_tb.data(), {1, nt, B, W, nxp}), mk(s_tb.data() + tb, {1, nt, B, W, nxp}),
+                    mk(s_lr.data(), {1, nt, B, nzp, W}), mk(s_lr.data() + lr, {1, nt, B, nzp, W})};
+        }
+        const Buf last_b = mk(last.data(), {1, 2, B, 1, nz, nx});
+        f.save_all_wavefields = res != nullptr && !use_bs;
+        f.use_boundary_saving = use_bs;
+        if (use_bs) {
+            f.boundary_gpu = list(bgpu);
+            f.last_two = last_b;
+        }
         f.nt = unsigned(nt);
         f.dt = dt;
         f.spacing = FloatSpan{spacing.data(), int64_t(spacing.size())};
         f.record_out = mk(rec_out, {B, nrec, nt});
-        if (res) f.u_allt_out = mk(u_allt.data(), ashape);
+        if (res && !use_bs) f.u_allt_out = mk(u_allt.data(), ashape);
         if (NDIM == 2) acoustic2d::forward_core(f);
         else acoustic3d::forward_core(f);
         if (!res) return 0;
 
         std::vector<Buf> grads{mk(grad_w, {B, nsrc, nt}), mk(grad_vp, gshape)};
         BackwardInputCore bi{};
-        bi.u_forward = mk(u_allt.data(), ashape);
+        std::vector<std::vector<float>> recon(use_bs ? 3 : 0, std::vector<float>(size_t(B * n), 0.f));
+        std::vector<Buf> rw;
+        for (auto &r : recon) rw.push_back(mk(r.data(), gshape));
+        if (use_bs) {
+            bi.u_last_two = last_b;
+            bi.boundary_gpu = list(bgpu);
+            bi.forward_wavefields = list(rw);
+        } else {
+            bi.u_forward = mk(u_allt.data(), ashape);
+        }
         bi.adjoint_wavefields = list(aw);
         bi.models = list(models);
         bi.adjoint_source = mk(res, {B, nrec, nt});
@@ -131,7 +158,8 @@ int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *
         bi.spacing = f.spacing;
         bi.compute_illumination = false;
         bi.grads_out = list(grads);
-        if (NDIM == 2) acoustic2d::backward_core(bi);
+        if (use_bs) acoustic2d::backward_bs_core(bi);
+        else if (NDIM == 2) acoustic2d::backward_core(bi);
         else acoustic3d::backward_core(bi);
         return 0;
     } catch (const std::exception &e) {
@@ -278,6 +306,16 @@ __attribute__((visibility("default"))) int ref_elastic_run_bs(
 {
     return run_elastic(grid, B, M, nt, abcn, dt, spacing_zx, pml, vp, vs, rho, grad, src, sloc, nsrc, rloc, nrec,
                        res, rec_out, g_vp, g_vs, g_rho, err, cap, 1);
+}
+
+// 2-D acoustic with boundary saving (backward_bs_core).
+__attribute__((visibility("default"))) int ref_acoustic2d_run_bs(
+    const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zx, const float *const *pml,
+    const float *vp, const float *lap, const float *grad, const float *src, const int *sloc, int nsrc,
+    const int *rloc, int nrec, const float *res, float *rec_out, float *grad_vp, float *grad_w, char *err, int cap)
+{
+    return run<2>(grid, B, M, nt, abcn, dt, spacing_zx, pml, vp, lap, grad, src, sloc, nsrc, rloc, nrec, res,
+                  rec_out, grad_vp, grad_w, err, cap, 1);
 }
 
 __attribute__((visibility("default"))) int ref_acoustic_run(

@@ -33,6 +33,7 @@ def lib():
         L.ref_acoustic_run.restype = ctypes.c_int
         L.ref_elastic_run.restype = ctypes.c_int
         L.ref_elastic_run_bs.restype = ctypes.c_int
+        L.ref_acoustic2d_run_bs.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -45,9 +46,10 @@ def _fp(a):
     return a.ctypes.data_as(_F) if a is not None else None
 
 
-def run(P, vp_pad, wav, sloc, rloc, res=None):
-    """Forward (and, with ``res``, the full-storage backward) of the reference
-    for problem ``P`` (an ``A2DProblem`` / ``A3DProblem``).  Returns
+def run(P, vp_pad, wav, sloc, rloc, res=None, bs=False):
+    """Forward (and, with ``res``, the full-storage backward -- the
+    boundary-saving one with ``bs``, 2-D only) of the reference for problem
+    ``P`` (an ``A2DProblem`` / ``A3DProblem``).  Returns
     ``dict(rec, grad_vp, grad_wavelet)`` on the runtime grid."""
     ndim = 3 if hasattr(P, "ny") else 2
     grid = np.array([P.nz, P.ny, P.nx] if ndim == 3 else [P.nz, P.nx], dtype=np.int32)
@@ -63,10 +65,18 @@ def run(P, vp_pad, wav, sloc, rloc, res=None):
     gv = np.zeros(vp_pad.shape, np.float32)
     gw = np.zeros((B, nsrc, nt), np.float32)
     err = ctypes.create_string_buffer(1024)
-    rc = lib().ref_acoustic_run(ndim, grid.ctypes.data_as(_I), B, P.M, nt, P.abcn, ctypes.c_float(P.dt),
-                                _fp(sp), parr, _fp(vp_pad), _fp(P.lap), _fp(P.grad), _fp(wav),
-                                sloc.ctypes.data_as(_I), nsrc, rloc.ctypes.data_as(_I), nrec, _fp(res),
-                                _fp(rec), _fp(gv), _fp(gw), err, 1024)
+    if bs:
+        if ndim != 2:
+            raise ValueError("the harness runs the boundary-saving backward in 2-D only")
+        rc = lib().ref_acoustic2d_run_bs(grid.ctypes.data_as(_I), B, P.M, nt, P.abcn, ctypes.c_float(P.dt),
+                                         _fp(sp), parr, _fp(vp_pad), _fp(P.lap), _fp(P.grad), _fp(wav),
+                                         sloc.ctypes.data_as(_I), nsrc, rloc.ctypes.data_as(_I), nrec, _fp(res),
+                                         _fp(rec), _fp(gv), _fp(gw), err, 1024)
+    else:
+        rc = lib().ref_acoustic_run(ndim, grid.ctypes.data_as(_I), B, P.M, nt, P.abcn, ctypes.c_float(P.dt),
+                                    _fp(sp), parr, _fp(vp_pad), _fp(P.lap), _fp(P.grad), _fp(wav),
+                                    sloc.ctypes.data_as(_I), nsrc, rloc.ctypes.data_as(_I), nrec, _fp(res),
+                                    _fp(rec), _fp(gv), _fp(gw), err, 1024)
     if rc != 0:
         raise RuntimeError("reference core: " + err.value.decode(errors="replace"))
     return dict(rec=rec, grad_vp=gv, grad_wavelet=gw)

@@ -5,6 +5,7 @@ tree is not present on GPU machines)::
 
     cd /tmp && python /root/repo/tests/golden/gen_refcore.py           # every fixture
     cd /tmp && python /root/repo/tests/golden/gen_refcore.py e2d_bs    # the elastic boundary-saving ones
+    cd /tmp && python /root/repo/tests/golden/gen_refcore.py a2d_bs    # the 2-D acoustic boundary-saving ones
 
 The reference's CPU host core (its own CUDA sources compiled as C++, see
 ``oracle/build_ref.sh``) runs each case in full-storage mode through
@@ -12,7 +13,8 @@ The reference's CPU host core (its own CUDA sources compiled as C++, see
 seeded residual, giving the model gradient on the runtime grid and the
 wavelet gradient.  The elastic cases are also run with boundary saving
 (``refcore_e2d_bs_*``: the reference's backward_bs_core rebuilding the
-forward state from its saved strips).  Problem assembly (padding, coordinate shift, CPML
+forward state from its saved strips), and the 2-D acoustic cases too
+(``refcore2d_bs_*``).  Problem assembly (padding, coordinate shift, CPML
 profiles, FD tables) is the oracle's, whose profiles and tables the table
 tests pin bit-exactly against the reference's own builders.  Only inputs and
 outputs are stored, as ``.npz`` arrays.
@@ -70,7 +72,8 @@ def main(only=None):
     if not refcore.available():
         raise SystemExit("oracle/_ref is not built: run oracle/build_ref.sh first")
     dh, dt, abcn, f0 = 10.0, 1e-3, 10, 15.0
-    for ndim, cases in (() if only == "e2d_bs" else ((2, CASES_2D), (3, CASES_3D))):
+    acoustic = {None: ((2, CASES_2D), (3, CASES_3D)), "a2d_bs": ((2, CASES_2D),)}.get(only, ())
+    for ndim, cases in acoustic:
         for name, order, shape, nsrc, B, nt in cases:
             rng = np.random.default_rng(1000 * ndim + order)
             ab = abcn if ndim == 2 else 8
@@ -96,14 +99,15 @@ def main(only=None):
             rec[..., -1] = 2
             wavB = np.broadcast_to(wav, (B, nsrc, nt)).copy()
             res = rng.standard_normal((B, nrec, nt)).astype(np.float32)
-            out = refcore.run(P, P.pad_model(vp), wavB, P.shift(src), P.shift(rec), res)
-            path = OUT / f"refcore{ndim}d_{name}.npz"
-            np.savez_compressed(path, order=order, shape=np.array(shape), dh=dh, dt=dt, nt=nt, abcn=ab, B=B,
-                                vp=vp, wavelet=wav, sources=src, receivers=rec, residual=res,
-                                rec=out["rec"], grad_vp=out["grad_vp"], grad_wavelet=out["grad_wavelet"])
+            for bs in ((False, True) if only is None and ndim == 2 else (only == "a2d_bs",)):
+                out = refcore.run(P, P.pad_model(vp), wavB, P.shift(src), P.shift(rec), res, bs=bs)
+                path = OUT / (f"refcore{ndim}d_bs_{name}.npz" if bs else f"refcore{ndim}d_{name}.npz")
+                np.savez_compressed(path, order=order, shape=np.array(shape), dh=dh, dt=dt, nt=nt, abcn=ab, B=B,
+                                    vp=vp, wavelet=wav, sources=src, receivers=rec, residual=res,
+                                    rec=out["rec"], grad_vp=out["grad_vp"], grad_wavelet=out["grad_wavelet"])
             print(path.name, os.path.getsize(path))
     # elastic 2-D: vz sources, vx + vz receivers, layered vp with vs = vp / 1.8, Gardner rho
-    modes = (True,) if only == "e2d_bs" else (False, True)
+    modes = {None: (False, True), "e2d_bs": (True,)}.get(only, ())
     dh, dt, abcn, f0 = 5.0, 5e-4, 10, 15.0
     for name, order, shape, B, nt in CASES_E2D:
         rng = np.random.default_rng(3000 + order)

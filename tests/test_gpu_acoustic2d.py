@@ -63,6 +63,37 @@ def _run(z, mode, dev, illumination=False):
     return out
 
 
+@pytest.mark.parametrize("case", ["refcore2d_bs_o2", "refcore2d_bs_o4", "refcore2d_bs_o8"])
+def test_boundary_saving_matches_reference_core_fixture(case):
+    """GPU boundary saving against the reference core's own boundary-saving
+    backward (backward_bs_core, fixtures from tests/golden/gen_refcore.py):
+    the adjoint source is the fixture's residual."""
+    dev = _dev()
+    print(f"[acoustic2d] fixture={case} mode=bs", flush=True)
+    z = np.load(GOLDEN / f"{case}.npz")
+    from oracle.oracle import A2DProblem
+    from sweep_amd.equations import Acoustic
+    from sweep_amd.propagator import PropTorch
+    B, nt = int(z["B"]), int(z["nt"])
+    P = A2DProblem(int(z["order"]), tuple(int(v) for v in z["shape"]), float(z["dh"]), float(z["dt"]), nt,
+                   abcn=int(z["abcn"]), B=B)
+    solver = PropTorch(Acoustic(spatial_order=int(z["order"]), device=dev), shape=tuple(int(s) for s in z["shape"]),
+                       dh=float(z["dh"]), dt=float(z["dt"]), device=dev, abcn=int(z["abcn"]), memory=_memory("bs"))
+    m = torch.tensor(z["vp"], device=dev, requires_grad=True)
+    w = torch.tensor(np.broadcast_to(z["wavelet"], (B, nt)).copy(), device=dev, requires_grad=True)
+    pred = solver(w, z["sources"][:, 0, :], z["receivers"], models=[m])
+    res = np.ascontiguousarray(np.asarray(z["residual"]).transpose(0, 2, 1)[..., None])
+    (pred * torch.tensor(res, device=dev)).sum().backward()
+    torch.cuda.synchronize()
+    e_rec = rel_l2(pred.detach().cpu().numpy()[..., 0].transpose(0, 2, 1), z["rec"])
+    e_g = rel_l2(m.grad.cpu().numpy(), P.crop(z["grad_vp"]).sum(0))
+    e_w = rel_l2(w.grad.cpu().numpy(), z["grad_wavelet"][:, 0])
+    print(f"[acoustic2d] {case} rec {e_rec:.2e} grad_vp {e_g:.2e} grad_w {e_w:.2e}", flush=True)
+    assert e_rec <= 1e-5
+    assert e_g <= 1e-4
+    assert e_w <= 1e-4
+
+
 _oracle_cache = {}
 
 

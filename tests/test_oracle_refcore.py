@@ -34,7 +34,8 @@ from conftest import GOLDEN, rel_l2
 oracle = pytest.importorskip("oracle.oracle")
 from oracle import refcore  # noqa: E402
 
-CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(str(GOLDEN / "refcore[23]d_*.npz")))
+CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(str(GOLDEN / "refcore[23]d_o*.npz")))
+BS2D_CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(str(GOLDEN / "refcore2d_bs_*.npz")))
 
 
 def problem(z):
@@ -71,6 +72,19 @@ def test_oracle_matches_reference_core(case, mode):
         assert np.abs(o["grad_wavelet"][1:]).max() > 0
     else:
         assert rel_l2(o["grad_wavelet"], z["grad_wavelet"]) <= bar_grad
+
+
+@pytest.mark.parametrize("case", BS2D_CASES)
+def test_oracle_boundary_saving_matches_reference_core(case):
+    """The oracle's boundary-saving backward against the reference's own
+    (backward_bs_core through oracle/ref_harness.cpp): both rebuild the
+    forward wavefield from the saved strips; measured <= 5.6e-6."""
+    z = np.load(GOLDEN / f"{case}.npz")
+    P, vp, wav, s, r = problem(z)
+    o = P.gradient(1, vp, wav, s, r, z["residual"])
+    assert rel_l2(o["rec"], z["rec"]) <= 1e-6
+    assert rel_l2(o["grad_vp"], z["grad_vp"]) <= 1e-5
+    assert rel_l2(o["grad_wavelet"], z["grad_wavelet"]) <= 1e-5
 
 
 @pytest.mark.parametrize("case", [c for c in CASES if c.startswith("refcore3d")])
