@@ -99,20 +99,33 @@ int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *
         f.receiver_field_indices = IntSpan{&fidx, 1};
         f.pml_vals = list(pml_b);
         f.wavefields = list(fw);
-        // boundary saving (2-D): the saver stores u only, strips M + 1 wide at
-        // offset 0, top / bottom (1, nt, B, W, nx_phys), left / right
-        // (1, nt, B, nz_phys, W); last state (1, 2, B, 1, nz, nx)
+        // boundary saving: the saver stores u only, strips M + 1 wide at
+        // offset 0.  2-D: top / bottom (1, nt, B, W, nx_phys), left / right
+        // (1, nt, B, nz_phys, W).  3-D (time first, the per-step stride):
+        // top / bottom (nt, B, W, ny_phys, nx_phys), front / back
+        // (nt, B, nz_phys, W, nx_phys), left / right (nt, B, nz_phys, ny_phys, W).
+        // Last state (1, 2, B, 1, grid).
         const int W = M + 1, nxp = nx - 2 * (abcn + M), nzp = nz - 2 * (abcn + M);
-        const bool use_bs = bs && NDIM == 2 && res;
-        std::vector<float> s_tb(use_bs ? size_t(2) * nt * B * W * nxp : 0), s_lr(use_bs ? size_t(2) * nt * B * nzp * W : 0);
+        const int nyp = NDIM == 3 ? ny - 2 * (abcn + M) : 1;
+        const bool use_bs = bs && res;
+        const int64_t ftb = int64_t(nt) * B * W * nyp * nxp, ffb = int64_t(nt) * B * nzp * W * nxp,
+                      flr = int64_t(nt) * B * nzp * nyp * W;
+        std::vector<float> s_tb(use_bs ? size_t(2 * ftb) : 0), s_fb(use_bs && NDIM == 3 ? size_t(2 * ffb) : 0),
+            s_lr(use_bs ? size_t(2 * flr) : 0);
         std::vector<float> last(use_bs ? size_t(2) * B * n : 0, 0.f);
         std::vector<Buf> bgpu;
-        if (use_bs) {
-            const int64_t tb = int64_t(nt) * B * W * nxp, lr = int64_t(nt) * B * nzp * W;
-            bgpu = {mk(s_tb.data(), {1, nt, B, W, nxp}), mk(s_tb.data() + tb, {1, nt, B, W, nxp}),
-                    mk(s_lr.data(), {1, nt, B, nzp, W}), mk(s_lr.data() + lr, {1, nt, B, nzp, W})};
+        if (use_bs && NDIM == 2) {
+            bgpu = {mk(s_tb.data(), {1, nt, B, W, nxp}), mk(s_tb.data() + ftb, {1, nt, B, W, nxp}),
+                    mk(s_lr.data(), {1, nt, B, nzp, W}), mk(s_lr.data() + flr, {1, nt, B, nzp, W})};
+        } else if (use_bs) {
+            bgpu = {mk(s_tb.data(), {nt, B, W, nyp, nxp}), mk(s_tb.data() + ftb, {nt, B, W, nyp, nxp}),
+                    mk(s_fb.data(), {nt, B, nzp, W, nxp}), mk(s_fb.data() + ffb, {nt, B, nzp, W, nxp}),
+                    mk(s_lr.data(), {nt, B, nzp, nyp, W}), mk(s_lr.data() + flr, {nt, B, nzp, nyp, W})};
         }
-        const Buf last_b = mk(last.data(), {1, 2, B, 1, nz, nx});
+        std::vector<int64_t> lshape{1, 2, B, 1, nz};
+        if (NDIM == 3) lshape.push_back(ny);
+        lshape.push_back(nx);
+        const Buf last_b = mk(last.data(), lshape);
         f.save_all_wavefields = res != nullptr && !use_bs;
         f.use_boundary_saving = use_bs;
         if (use_bs) {
@@ -158,7 +171,8 @@ int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *
         bi.spacing = f.spacing;
         bi.compute_illumination = false;
         bi.grads_out = list(grads);
-        if (use_bs) acoustic2d::backward_bs_core(bi);
+        if (use_bs && NDIM == 2) acoustic2d::backward_bs_core(bi);
+        else if (use_bs) acoustic3d::backward_bs_core(bi);
         else if (NDIM == 2) acoustic2d::backward_core(bi);
         else acoustic3d::backward_core(bi);
         return 0;
@@ -315,6 +329,16 @@ __attribute__((visibility("default"))) int ref_acoustic2d_run_bs(
     const int *rloc, int nrec, const float *res, float *rec_out, float *grad_vp, float *grad_w, char *err, int cap)
 {
     return run<2>(grid, B, M, nt, abcn, dt, spacing_zx, pml, vp, lap, grad, src, sloc, nsrc, rloc, nrec, res,
+                  rec_out, grad_vp, grad_w, err, cap, 1);
+}
+
+// 3-D acoustic with boundary saving (backward_bs_core).
+__attribute__((visibility("default"))) int ref_acoustic3d_run_bs(
+    const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zyx, const float *const *pml,
+    const float *vp, const float *lap, const float *grad, const float *src, const int *sloc, int nsrc,
+    const int *rloc, int nrec, const float *res, float *rec_out, float *grad_vp, float *grad_w, char *err, int cap)
+{
+    return run<3>(grid, B, M, nt, abcn, dt, spacing_zyx, pml, vp, lap, grad, src, sloc, nsrc, rloc, nrec, res,
                   rec_out, grad_vp, grad_w, err, cap, 1);
 }
 

@@ -34,6 +34,7 @@ def lib():
         L.ref_elastic_run.restype = ctypes.c_int
         L.ref_elastic_run_bs.restype = ctypes.c_int
         L.ref_acoustic2d_run_bs.restype = ctypes.c_int
+        L.ref_acoustic3d_run_bs.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -48,7 +49,7 @@ def _fp(a):
 
 def run(P, vp_pad, wav, sloc, rloc, res=None, bs=False):
     """Forward (and, with ``res``, the full-storage backward -- the
-    boundary-saving one with ``bs``, 2-D only) of the reference for problem
+    boundary-saving one with ``bs``) of the reference for problem
     ``P`` (an ``A2DProblem`` / ``A3DProblem``).  Returns
     ``dict(rec, grad_vp, grad_wavelet)`` on the runtime grid."""
     ndim = 3 if hasattr(P, "ny") else 2
@@ -66,9 +67,8 @@ def run(P, vp_pad, wav, sloc, rloc, res=None, bs=False):
     gw = np.zeros((B, nsrc, nt), np.float32)
     err = ctypes.create_string_buffer(1024)
     if bs:
-        if ndim != 2:
-            raise ValueError("the harness runs the boundary-saving backward in 2-D only")
-        rc = lib().ref_acoustic2d_run_bs(grid.ctypes.data_as(_I), B, P.M, nt, P.abcn, ctypes.c_float(P.dt),
+        fn = lib().ref_acoustic2d_run_bs if ndim == 2 else lib().ref_acoustic3d_run_bs
+        rc = fn(grid.ctypes.data_as(_I), B, P.M, nt, P.abcn, ctypes.c_float(P.dt),
                                          _fp(sp), parr, _fp(vp_pad), _fp(P.lap), _fp(P.grad), _fp(wav),
                                          sloc.ctypes.data_as(_I), nsrc, rloc.ctypes.data_as(_I), nrec, _fp(res),
                                          _fp(rec), _fp(gv), _fp(gw), err, 1024)

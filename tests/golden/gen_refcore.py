@@ -6,6 +6,7 @@ tree is not present on GPU machines)::
     cd /tmp && python /root/repo/tests/golden/gen_refcore.py           # every fixture
     cd /tmp && python /root/repo/tests/golden/gen_refcore.py e2d_bs    # the elastic boundary-saving ones
     cd /tmp && python /root/repo/tests/golden/gen_refcore.py a2d_bs    # the 2-D acoustic boundary-saving ones
+    cd /tmp && python /root/repo/tests/golden/gen_refcore.py a3d_bs    # the 3-D acoustic boundary-saving ones
 
 The reference's CPU host core (its own CUDA sources compiled as C++, see
 ``oracle/build_ref.sh``) runs each case in full-storage mode through
@@ -13,8 +14,8 @@ The reference's CPU host core (its own CUDA sources compiled as C++, see
 seeded residual, giving the model gradient on the runtime grid and the
 wavelet gradient.  The elastic cases are also run with boundary saving
 (``refcore_e2d_bs_*``: the reference's backward_bs_core rebuilding the
-forward state from its saved strips), and the 2-D acoustic cases too
-(``refcore2d_bs_*``).  Problem assembly (padding, coordinate shift, CPML
+forward state from its saved strips), and the acoustic cases too
+(``refcore{2,3}d_bs_*``).  Problem assembly (padding, coordinate shift, CPML
 profiles, FD tables) is the oracle's, whose profiles and tables the table
 tests pin bit-exactly against the reference's own builders.  Only inputs and
 outputs are stored, as ``.npz`` arrays.
@@ -72,7 +73,8 @@ def main(only=None):
     if not refcore.available():
         raise SystemExit("oracle/_ref is not built: run oracle/build_ref.sh first")
     dh, dt, abcn, f0 = 10.0, 1e-3, 10, 15.0
-    acoustic = {None: ((2, CASES_2D), (3, CASES_3D)), "a2d_bs": ((2, CASES_2D),)}.get(only, ())
+    acoustic = {None: ((2, CASES_2D), (3, CASES_3D)), "a2d_bs": ((2, CASES_2D),),
+                "a3d_bs": ((3, CASES_3D),)}.get(only, ())
     for ndim, cases in acoustic:
         for name, order, shape, nsrc, B, nt in cases:
             rng = np.random.default_rng(1000 * ndim + order)
@@ -99,7 +101,7 @@ def main(only=None):
             rec[..., -1] = 2
             wavB = np.broadcast_to(wav, (B, nsrc, nt)).copy()
             res = rng.standard_normal((B, nrec, nt)).astype(np.float32)
-            for bs in ((False, True) if only is None and ndim == 2 else (only == "a2d_bs",)):
+            for bs in ((False, True) if only is None else (True,)):
                 out = refcore.run(P, P.pad_model(vp), wavB, P.shift(src), P.shift(rec), res, bs=bs)
                 path = OUT / (f"refcore{ndim}d_bs_{name}.npz" if bs else f"refcore{ndim}d_{name}.npz")
                 np.savez_compressed(path, order=order, shape=np.array(shape), dh=dh, dt=dt, nt=nt, abcn=ab, B=B,

@@ -18,6 +18,7 @@ from conftest import GOLDEN, rel_l2
 pytestmark = pytest.mark.gpu
 
 FIXTURES = ["refcore3d_o2", "refcore3d_o4", "refcore3d_o8"]
+BS_FIXTURES = ["refcore3d_bs_o2", "refcore3d_bs_o4", "refcore3d_bs_o8"]
 
 
 def _dev():
@@ -64,9 +65,13 @@ def _gpu_run(order, shape, dh, dt, abcn, mode, vp, wav, src, rec, res, dev, per_
     return out
 
 
-@pytest.mark.parametrize("case", FIXTURES)
+@pytest.mark.parametrize("case", FIXTURES + BS_FIXTURES)
 @pytest.mark.parametrize("mode", ["full", "bs", "ckpt"])
 def test_matches_reference_core_fixture(case, mode):
+    """Full-mode fixtures for every memory mode; the boundary-saving fixtures
+    (the reference's own backward_bs_core) for the boundary-saving mode."""
+    if "_bs_" in case and mode != "bs":
+        pytest.skip("the boundary-saving fixtures pin the boundary-saving mode")
     dev = _dev()
     print(f"[acoustic3d] fixture={case} mode={mode}", flush=True)
     z = np.load(GOLDEN / f"{case}.npz")
