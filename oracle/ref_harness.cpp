@@ -55,7 +55,8 @@ template <int NDIM>
 int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zyx,
         const float *const *pml, const float *vp, const float *lap, const float *grad, const float *src,
         const int *sloc, int nsrc, const int *rloc, int nrec, const float *res, float *rec_out,
-        float *grad_vp, float *grad_w, char *err, int cap, int bs = 0)
+        float *grad_vp, float *grad_w, char *err, int cap, int bs = 0, float *illum_s = nullptr,
+        float *illum_r = nullptr)
 {
     try {
         const int nz = grid[0], ny = NDIM == 3 ? grid[1] : 1, nx = grid[NDIM - 1];
@@ -169,7 +170,10 @@ int run(const int *grid, int B, int M, int nt, int abcn, float dt, const float *
         bi.nt = unsigned(nt);
         bi.dt = dt;
         bi.spacing = f.spacing;
-        bi.compute_illumination = false;
+        std::vector<Buf> illum;
+        if (illum_s && illum_r) illum = {mk(illum_s, gshape), mk(illum_r, gshape)};
+        bi.compute_illumination = !illum.empty();
+        if (!illum.empty()) bi.illum_out = list(illum);
         bi.grads_out = list(grads);
         if (use_bs && NDIM == 2) acoustic2d::backward_bs_core(bi);
         else if (use_bs) acoustic3d::backward_bs_core(bi);
@@ -320,6 +324,21 @@ __attribute__((visibility("default"))) int ref_elastic_run_bs(
 {
     return run_elastic(grid, B, M, nt, abcn, dt, spacing_zx, pml, vp, vs, rho, grad, src, sloc, nsrc, rloc, nrec,
                        res, rec_out, g_vp, g_vs, g_rho, err, cap, 1);
+}
+
+// 2-D / 3-D acoustic with the source and receiver illumination (sum u_tt^2,
+// sum lambda^2, (B, grid) each), full storage (bs = 0) or boundary saving.
+__attribute__((visibility("default"))) int ref_acoustic_run_illum(
+    int ndim, int bs, const int *grid, int B, int M, int nt, int abcn, float dt, const float *spacing_zyx,
+    const float *const *pml, const float *vp, const float *lap, const float *grad, const float *src,
+    const int *sloc, int nsrc, const int *rloc, int nrec, const float *res, float *rec_out, float *grad_vp,
+    float *grad_w, float *illum_s, float *illum_r, char *err, int cap)
+{
+    if (ndim == 2)
+        return run<2>(grid, B, M, nt, abcn, dt, spacing_zyx, pml, vp, lap, grad, src, sloc, nsrc, rloc, nrec, res,
+                      rec_out, grad_vp, grad_w, err, cap, bs, illum_s, illum_r);
+    return run<3>(grid, B, M, nt, abcn, dt, spacing_zyx, pml, vp, lap, grad, src, sloc, nsrc, rloc, nrec, res,
+                  rec_out, grad_vp, grad_w, err, cap, bs, illum_s, illum_r);
 }
 
 // 2-D acoustic with boundary saving (backward_bs_core).

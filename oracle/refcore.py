@@ -35,6 +35,7 @@ def lib():
         L.ref_elastic_run_bs.restype = ctypes.c_int
         L.ref_acoustic2d_run_bs.restype = ctypes.c_int
         L.ref_acoustic3d_run_bs.restype = ctypes.c_int
+        L.ref_acoustic_run_illum.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -45,6 +46,33 @@ def _f32(a):
 
 def _fp(a):
     return a.ctypes.data_as(_F) if a is not None else None
+
+
+def run_illum(P, vp_pad, wav, sloc, rloc, res, bs=False):
+    """The reference's backward with compute_illumination: ``dict(rec,
+    grad_vp, grad_wavelet, illum_s, illum_r)`` on the runtime grid."""
+    ndim = 3 if hasattr(P, "ny") else 2
+    grid = np.array([P.nz, P.ny, P.nx] if ndim == 3 else [P.nz, P.nx], dtype=np.int32)
+    sp = np.array([P.dh_zyx[k] for k in range(ndim)], dtype=np.float32)
+    B, nt = P.B, P.nt
+    nsrc, nrec = sloc.shape[1], rloc.shape[1]
+    prof = [_f32(p) for p in P.prof]
+    parr = (_F * len(prof))(*[_fp(p) for p in prof])
+    vp_pad, wav, res = _f32(vp_pad), _f32(wav), _f32(res)
+    sloc, rloc = np.ascontiguousarray(sloc, np.int32), np.ascontiguousarray(rloc, np.int32)
+    rec = np.zeros((B, nrec, nt), np.float32)
+    gv = np.zeros(vp_pad.shape, np.float32)
+    gw = np.zeros((B, nsrc, nt), np.float32)
+    S = np.zeros(vp_pad.shape, np.float32)
+    R = np.zeros(vp_pad.shape, np.float32)
+    err = ctypes.create_string_buffer(1024)
+    rc = lib().ref_acoustic_run_illum(ndim, 1 if bs else 0, grid.ctypes.data_as(_I), B, P.M, nt, P.abcn,
+                                      ctypes.c_float(P.dt), _fp(sp), parr, _fp(vp_pad), _fp(P.lap), _fp(P.grad),
+                                      _fp(wav), sloc.ctypes.data_as(_I), nsrc, rloc.ctypes.data_as(_I), nrec,
+                                      _fp(res), _fp(rec), _fp(gv), _fp(gw), _fp(S), _fp(R), err, 1024)
+    if rc != 0:
+        raise RuntimeError("reference core: " + err.value.decode(errors="replace"))
+    return dict(rec=rec, grad_vp=gv, grad_wavelet=gw, illum_s=S, illum_r=R)
 
 
 def run(P, vp_pad, wav, sloc, rloc, res=None, bs=False):

@@ -7,6 +7,7 @@ tree is not present on GPU machines)::
     cd /tmp && python /root/repo/tests/golden/gen_refcore.py e2d_bs    # the elastic boundary-saving ones
     cd /tmp && python /root/repo/tests/golden/gen_refcore.py a2d_bs    # the 2-D acoustic boundary-saving ones
     cd /tmp && python /root/repo/tests/golden/gen_refcore.py a3d_bs    # the 3-D acoustic boundary-saving ones
+    cd /tmp && python /root/repo/tests/golden/gen_refcore.py illum     # the illumination ones
 
 The reference's CPU host core (its own CUDA sources compiled as C++, see
 ``oracle/build_ref.sh``) runs each case in full-storage mode through
@@ -15,7 +16,8 @@ seeded residual, giving the model gradient on the runtime grid and the
 wavelet gradient.  The elastic cases are also run with boundary saving
 (``refcore_e2d_bs_*``: the reference's backward_bs_core rebuilding the
 forward state from its saved strips), and the acoustic cases too
-(``refcore{2,3}d_bs_*``).  Problem assembly (padding, coordinate shift, CPML
+(``refcore{2,3}d_bs_*``).  ``refcore{2,3}d_illum[_bs]_*`` add the source and
+receiver illumination (compute_illumination) of one case per dimension.  Problem assembly (padding, coordinate shift, CPML
 profiles, FD tables) is the oracle's, whose profiles and tables the table
 tests pin bit-exactly against the reference's own builders.  Only inputs and
 outputs are stored, as ``.npz`` arrays.
@@ -74,7 +76,8 @@ def main(only=None):
         raise SystemExit("oracle/_ref is not built: run oracle/build_ref.sh first")
     dh, dt, abcn, f0 = 10.0, 1e-3, 10, 15.0
     acoustic = {None: ((2, CASES_2D), (3, CASES_3D)), "a2d_bs": ((2, CASES_2D),),
-                "a3d_bs": ((3, CASES_3D),)}.get(only, ())
+                "a3d_bs": ((3, CASES_3D),), "illum": ((2, CASES_2D), (3, CASES_3D))}.get(only, ())
+    illum_cases = {(2, "o8"), (3, "o4")}
     for ndim, cases in acoustic:
         for name, order, shape, nsrc, B, nt in cases:
             rng = np.random.default_rng(1000 * ndim + order)
@@ -101,6 +104,17 @@ def main(only=None):
             rec[..., -1] = 2
             wavB = np.broadcast_to(wav, (B, nsrc, nt)).copy()
             res = rng.standard_normal((B, nrec, nt)).astype(np.float32)
+            if only in (None, "illum") and (ndim, name) in illum_cases:
+                for bs in (False, True):
+                    out = refcore.run_illum(P, P.pad_model(vp), wavB, P.shift(src), P.shift(rec), res, bs=bs)
+                    path = OUT / f"refcore{ndim}d_illum{'_bs' if bs else ''}_{name}.npz"
+                    np.savez_compressed(path, order=order, shape=np.array(shape), dh=dh, dt=dt, nt=nt, abcn=ab,
+                                        B=B, vp=vp, wavelet=wav, sources=src, receivers=rec, residual=res,
+                                        rec=out["rec"], grad_vp=out["grad_vp"], grad_wavelet=out["grad_wavelet"],
+                                        illum_s=out["illum_s"], illum_r=out["illum_r"])
+                    print(path.name, os.path.getsize(path))
+            if only == "illum":
+                continue
             for bs in ((False, True) if only is None else (True,)):
                 out = refcore.run(P, P.pad_model(vp), wavB, P.shift(src), P.shift(rec), res, bs=bs)
                 path = OUT / (f"refcore{ndim}d_bs_{name}.npz" if bs else f"refcore{ndim}d_{name}.npz")

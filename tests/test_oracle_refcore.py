@@ -101,6 +101,22 @@ def test_oracle3d_boundary_saving_matches_reference_core(case):
     assert rel_l2(o["grad_wavelet"][0], z["grad_wavelet"][0]) <= 1e-6
 
 
+ILLUM_CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(str(GOLDEN / "refcore[23]d_illum_*.npz")))
+
+
+@pytest.mark.parametrize("case", ILLUM_CASES)
+def test_oracle_illumination_matches_reference_core(case):
+    """Source (sum u_tt^2) and receiver (sum lambda^2) illumination against
+    the reference's compute_illumination, full storage and boundary saving;
+    measured <= 1.6e-8."""
+    z = np.load(GOLDEN / f"{case}.npz")
+    P, vp, wav, s, r = problem(z)
+    o = P.gradient(1 if "_bs_" in case else 0, vp, wav, s, r, z["residual"], illumination=True)
+    assert len(ILLUM_CASES) >= 4
+    assert rel_l2(o["source_illumination"], z["illum_s"]) <= 1e-6
+    assert rel_l2(o["receiver_illumination"], z["illum_r"]) <= 1e-6
+
+
 @pytest.mark.parametrize("case", [c for c in CASES if c.startswith("refcore3d")])
 def test_oracle3d_wavelet_adjoint_identity(case):
     """grad_wavelet = F^T r for the linear map w -> record: <F w, r> = <w, F^T r>."""
