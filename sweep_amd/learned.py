@@ -44,11 +44,13 @@ class Pair:
     target: torch.Tensor  # (H, W) float, (v_true - vp) / TARGET_SCALE
 
 
-def generate_pairs(make_fwi, models, iterations: int, device) -> list:
+def generate_pairs(make_fwi, models, iterations: int, device, log=None) -> list:
     """make_fwi(vp_true) -> (PlainFWI, vp_start); records `iterations` pairs
-    along each model's plain FWI path."""
+    along each model's plain FWI path (log(k, n) after each model)."""
     pairs = []
-    for vt in models:
+    for k, vt in enumerate(models):
+        if log is not None and k > 0:
+            log(k, len(models))
         fwi, vp = make_fwi(vt)
         vt = torch.as_tensor(vt, device=device)
         for _ in range(iterations):
@@ -137,6 +139,8 @@ def _main():
     ap.add_argument("--nx", type=int, default=1152)
     ap.add_argument("--nt", type=int, default=3000)
     ap.add_argument("--shots", type=int, default=32)
+    ap.add_argument("--pair-shots", type=int, default=None,
+                    help="shots per gradient when generating training pairs (default: --shots)")
     ap.add_argument("--train-models", type=int, default=8)
     ap.add_argument("--pair-iterations", type=int, default=6)
     ap.add_argument("--train-steps", type=int, default=400)
@@ -153,11 +157,15 @@ def _main():
     wav = synthetic.ricker(np.arange(args.nt) * dt - 1.5 / f0, f0)
     solver = PropTorch(Acoustic(spatial_order=8, device=dev), shape=shape, dh=dh, dt=dt, device=dev, abcn=50,
                        memory=BoundarySaving())
-    src, rec, mask = synthetic.surface_acquisition(shape, args.shots, seed=0)
-    batches = [(a, min(a + 32, args.shots)) for a in range(0, args.shots, 32)]
-    masks = [torch.tensor(mask[a:b], device=dev) for a, b in batches]
+    def acquisition(nshots):
+        src, rec, mask = synthetic.surface_acquisition(shape, nshots, seed=0)
+        batches = [(a, min(a + 32, nshots)) for a in range(0, nshots, 32)]
+        return src, rec, batches, [torch.tensor(mask[a:b], device=dev) for a, b in batches]
 
-    def make(vt_np, cls=PlainFWI, extra=(), **kw):
+    acq = {n: acquisition(n) for n in {args.shots, args.pair_shots or args.shots}}
+
+    def make(vt_np, cls=PlainFWI, extra=(), nshots=args.shots, **kw):
+        src, rec, batches, masks = acq[nshots]
         vt = torch.tensor(vt_np, device=dev)
         with torch.no_grad():
             obs = [solver(wav, src[a:b], rec[a:b], models=[vt]) for a, b in batches]
@@ -167,7 +175,10 @@ def _main():
 
     t0 = time.perf_counter()
     train_models = [synthetic.layered_2d(shape, seed=100 + k) for k in range(args.train_models)]
-    pairs = generate_pairs(make, train_models, args.pair_iterations, dev)
+    pairs = generate_pairs(lambda vt: make(vt, nshots=args.pair_shots or args.shots), train_models,
+                           args.pair_iterations, dev,
+                           log=lambda k, n: print(json.dumps({"stage": "pairs", "models_done": k, "of": n}),
+                                                  flush=True))
     torch.cuda.synchronize()
     print(json.dumps({"stage": "pairs", "count": len(pairs), "seconds": round(time.perf_counter() - t0, 2)}),
           flush=True)
