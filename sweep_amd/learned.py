@@ -139,14 +139,14 @@ def _main():
     ap.add_argument("--nx", type=int, default=1152)
     ap.add_argument("--nt", type=int, default=3000)
     ap.add_argument("--shots", type=int, default=32)
-    ap.add_argument("--pair-shots", type=int, default=None,
-                    help="shots per gradient when generating training pairs (default: --shots)")
-    ap.add_argument("--train-models", type=int, default=8)
-    ap.add_argument("--pair-iterations", type=int, default=6)
-    ap.add_argument("--train-steps", type=int, default=400)
-    ap.add_argument("--iterations", type=int, default=10)
+    ap.add_argument("--pair-shots", type=int, default=16,
+                    help="shots per gradient when generating training pairs")
+    ap.add_argument("--train-models", type=int, default=64)
+    ap.add_argument("--pair-iterations", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--iterations", type=int, default=8)
     ap.add_argument("--base", type=int, default=32)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--learned-scales", default="1,0.5,0.25",
                     help="comma-separated trial scales of the network's update, tried in order")
