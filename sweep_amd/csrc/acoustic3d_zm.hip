@@ -198,6 +198,7 @@ struct Lanes {
 // PlaneMap (rows j = w + 4t of the 2M rows above and below the tile).
 template <int M, int RY>
 struct XHalo {
+    static_assert(2 * M * RY <= 64, "x-halo duties exceed one wave");
     uint32_t off;   // in-plane byte offset (clamped into the grid)
     int col;        // global column (clamped)
     int lds;        // index in the LDS plane, -1: no duty
@@ -998,7 +999,10 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
 }
 
 // ---------------------------------------------------------------- host plans
-constexpr int kRyInt = 4;     // rows per lane of the interior kernels (tile 64 x 16)
+// Rows per lane of the interior kernels (a block's tile is 64 x 4*RY cells).
+constexpr int kRyFwd = 4;
+constexpr int kRyAdj = 4;
+constexpr int kRyRecon = 4;
 
 ZRect make_rect(int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, int px)
 {
@@ -1101,7 +1105,7 @@ template <int M, bool BX, bool BY, bool BZ>
 void fwd_region(const Ctx &x, const StepFields &f, float *u_this, float *strip)
 {
     constexpr bool SHELL = BX || BY || BZ;
-    constexpr int RY = SHELL ? ry_fwd<BX, BY, BZ>() : kRyInt;
+    constexpr int RY = SHELL ? ry_fwd<BX, BY, BZ>() : kRyFwd;
     const ZSet s = region_set(x.g, BX, BY, BZ, x.tz, 4 * RY);
     if (!s.per_shot) return;
     hipLaunchKernelGGL((fwd3_kernel<M, RY, BX, BY, BZ>), grid_of(s, x.g.B), dim3(256), 0, stream(), x.g, x.c, x.pr,
@@ -1112,7 +1116,7 @@ template <int M, bool BX, bool BY, bool BZ>
 void adj_region(const Ctx &x, const AdjFields &f, const float *img, float *grad)
 {
     constexpr bool SHELL = BX || BY || BZ;
-    constexpr int RY = SHELL ? ry_adj<BX, BY, BZ>() : kRyInt;
+    constexpr int RY = SHELL ? ry_adj<BX, BY, BZ>() : kRyAdj;
     const ZSet s = region_set(x.g, BX, BY, BZ, x.tz, 4 * RY);
     if (!s.per_shot) return;
     hipLaunchKernelGGL((adj3_kernel<M, RY, BX, BY, BZ>), grid_of(s, x.g.B), dim3(256), 0, stream(), x.g, x.c, x.pr,
@@ -1187,11 +1191,11 @@ void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const f
 {
     const Geo &g = x.g;
     ZSet s{};
-    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyInt, x.g.px);
+    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyRecon, x.g.px);
     if (!s.per_shot) return;
     by_order(g.M, [&](auto ms) {
         constexpr int M = decltype(ms)::value;
-        hipLaunchKernelGGL((recon_zm_kernel<M, kRyInt>), grid_of(s, g.B), dim3(256), 0, stream(), g, x.c, vp, u_prev,
+        hipLaunchKernelGGL((recon_zm_kernel<M, kRyRecon>), grid_of(s, g.B), dim3(256), 0, stream(), g, x.c, vp, u_prev,
                            u_now, u_next, adj, grad, s);
     });
     SWP_LAUNCHED();
