@@ -166,10 +166,15 @@ SWP_CAPI int64_t swp_revolve_spine(int32_t nt, int32_t slots, int32_t *pts, int6
 /* Launch-shape tuning (no reference counterpart: the reference fixes its
  * block shapes at compile time).  key: SWP_TUNE_*; value > 0 sets it, 0
  * restores the default (the SWP_TZ* environment variable if set, else the
- * built-in).  Applies to calls made after it, process-wide.  Records are
- * bit-identical under any setting, and so are 3-D gradients; a 2-D gradient
- * moves at the float32 rounding level (measured 4e-8 relative) when the
- * reconstruction segment height changes.  swp_set_tuning returns 0, or 1 for an unknown key or a
+ * built-in).  Applies to calls made after it, process-wide.  3-D records and
+ * gradients are bit-identical under any setting.  2-D results move at the
+ * float32 rounding level: gradients with the reconstruction segment height
+ * (measured 4e-8 relative), records where the interior segment height moves
+ * cells between the interior and band kernels (1.2e-6 at 96 x 128).  The
+ * 2-D heights act on launches of at least 4M shot-cells, which run the
+ * z-marching kernels (smaller ones run one thread per cell; the
+ * SWP_KERNELS environment variable, percell or zmarch, forces a family).
+ * swp_set_tuning returns 0, or 1 for an unknown key or a
  * negative value; swp_get_tuning returns the value set (0: default), -1 for
  * an unknown key. */
 enum {

@@ -947,11 +947,20 @@ Ctx make_ctx(const swp_problem &p)
     SWP_CHECK(x.g.X1 > x.g.X0 && x.g.Z1 > x.g.Z0, "acoustic2d: empty physical box");
     x.c = make_coef(p, p.lap_coes, p.grad_coes);
     x.pr = Prof{p.pml[0], p.pml[1], p.pml[2], p.pml[3], p.pml[4], p.pml[5]};
-    // SWP_KERNELS=percell keeps the one-thread-per-cell kernels (any order);
-    // the segment heights of the z-marching kernels come from the tuning
+    // Kernel family: the z-marching kernels from kZmMinCells cells per launch
+    // (B x nz x nx on the runtime grid) up, the one-thread-per-cell kernels
+    // (any order) below.  A z-march block walks its segment row by row, one
+    // load latency per row, so on a small launch the few blocks it makes
+    // leave the chip idle: at 96 x 128 and one shot it takes 84 us per step of
+    // forward + backward against 24 for the per-cell kernels; the two meet
+    // near 4.7M cells (147 against 160 us) and at 384 x 1152 x 32 shots the
+    // z-march is 2.8x faster.  SWP_KERNELS=percell / zmarch forces a family.
+    // The segment heights of the z-marching kernels come from the tuning
     // table (swp_set_tuning), else SWP_TZ / SWP_TZ_BAND / SWP_TZ_RECON.
     const char *kern = std::getenv("SWP_KERNELS");
-    x.zm = zm_supported(x.g) && !(kern && std::string(kern) == "percell");
+    const std::string ks = kern ? kern : "";
+    const int64_t cells = int64_t(x.g.B) * x.g.nz * x.g.nx;
+    x.zm = zm_supported(x.g) && ks != "percell" && (ks == "zmarch" || cells >= kZmMinCells);
     x.tz_recon = tuning(SWP_TUNE_TZ2_RECON, "SWP_TZ_RECON", 32);
     x.plan = make_plan(x.g, tuning(SWP_TUNE_TZ2, "SWP_TZ", 32), tuning(SWP_TUNE_TZ2_BAND, "SWP_TZ_BAND", 8));
     return x;

@@ -6,6 +6,10 @@
 namespace swp {
 namespace a2d {
 
+// Smallest launch (shots x runtime-grid cells) that runs on the z-marching
+// kernels by default; smaller ones run one thread per cell.
+constexpr int64_t kZmMinCells = int64_t(4) << 20;
+
 ZmPlan make_plan(const Geo &g, int tz, int tz_band);
 bool zm_supported(const Geo &g);
 void launch_fwd_zm(const Ctx &x, const ZmPlan &p, const StepFields &f, float *u_this, float *strip);

@@ -253,6 +253,8 @@ class PropTorch(torch.nn.Module):
 
     def _runtime_coords(self, loc):
         off = geometry.runtime_coord_offset(self.pad, self.equation.M, self.ndim)
+        if isinstance(loc, torch.Tensor):   # coordinates may live on the GPU
+            loc = loc.detach().cpu().numpy()
         loc = np.array(loc, dtype=np.int64, copy=True)
         for a in range(self.ndim):
             loc[..., a] += off[a]

@@ -22,6 +22,18 @@ CASES = ["c1_order4", "het_order8", "het_order6", "het_order2", "het_order10"]
 MODES = {"full": 0, "bs": 1, "ckpt": 0, "bs_bf16": 1}
 
 
+@pytest.fixture(autouse=True, params=["auto", "zmarch"])
+def kernels(request, monkeypatch):
+    """Every case runs on both 2-D kernel families: the automatic choice (the
+    one-thread-per-cell kernels at these sizes, below the z-march threshold)
+    and the z-marching kernels forced with SWP_KERNELS."""
+    if request.param == "auto":
+        monkeypatch.delenv("SWP_KERNELS", raising=False)
+    else:
+        monkeypatch.setenv("SWP_KERNELS", request.param)
+    return request.param
+
+
 def _dev():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")

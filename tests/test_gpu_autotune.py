@@ -31,7 +31,10 @@ def _grad(solver, wav, src, rec, vp):
 
 
 @pytest.fixture(autouse=True)
-def _reset():
+def _reset(monkeypatch):
+    # the 2-D cases here are small enough for the per-cell kernels: force
+    # the z-marching ones, whose segment heights are what is tuned
+    monkeypatch.setenv("SWP_KERNELS", "zmarch")
     yield
     for k in (_core.TUNE_TZ2, _core.TUNE_TZ2_BAND, _core.TUNE_TZ2_RECON, _core.TUNE_TZ3):
         _core.set_tuning(k, 0)
