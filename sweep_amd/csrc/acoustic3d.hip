@@ -940,8 +940,16 @@ Ctx make_ctx(const swp_problem &p)
         x.pr.db[a] = p.pml[3 * src + 2];
     }
     x.tz = tuning(SWP_TUNE_TZ3, "SWP_TZ3", 64);
+    // Kernel family, as in 2-D: the z-marching kernels from kZmMinCells3
+    // shot-cells (B x nz x ny x nx on the runtime grid) up, one thread per
+    // cell below, where the few long z-march segments of a small launch are
+    // latency-bound (40 x 36 x 44, one shot: 672 against 37 us per step of
+    // forward + backward; the two meet near 41M shot-cells).
+    // SWP_KERNELS=percell / zmarch forces a family.
     const char *kern = std::getenv("SWP_KERNELS");
-    x.zm = zm_supported(x.g, x.tz) && !(kern && std::string(kern) == "percell");
+    const std::string ks = kern ? kern : "";
+    const int64_t cells = int64_t(x.g.B) * x.g.nz * x.g.ny * x.g.nx;
+    x.zm = zm_supported(x.g, x.tz) && ks != "percell" && (ks == "zmarch" || cells >= kZmMinCells3);
     // SWP_SHELL3=percell runs the CPML shell with the per-cell kernels over
     // six slabs instead of the region-specialised z-marching kernels.
     const char *sh = std::getenv("SWP_SHELL3");

@@ -8,6 +8,10 @@ namespace a3d {
 
 // Orders 2..8, and a z segment of tz planes plus its 2M + 1 halo and prefetch
 // planes addressable with 32-bit byte offsets (tz is clamped to fit).
+// Smallest launch (shots x runtime-grid cells) that runs on the z-marching
+// kernels by default; smaller ones run one thread per cell.
+constexpr int64_t kZmMinCells3 = int64_t(32) << 20;
+
 bool zm_supported(const Geo &g, int &tz);
 // The slabs of the computed box around the lean interior [lo + M + 1,
 // hi - M - 1) per axis (the whole box when that interior is empty); returns

@@ -21,6 +21,18 @@ FIXTURES = ["refcore3d_o2", "refcore3d_o4", "refcore3d_o8"]
 BS_FIXTURES = ["refcore3d_bs_o2", "refcore3d_bs_o4", "refcore3d_bs_o8"]
 
 
+@pytest.fixture(autouse=True, params=["auto", "zmarch"])
+def kernels(request, monkeypatch):
+    """Every case runs on both 3-D kernel families: the automatic choice (the
+    one-thread-per-cell kernels at these sizes, below the z-march threshold)
+    and the region-specialised z-marching kernels forced with SWP_KERNELS."""
+    if request.param == "auto":
+        monkeypatch.delenv("SWP_KERNELS", raising=False)
+    else:
+        monkeypatch.setenv("SWP_KERNELS", request.param)
+    return request.param
+
+
 def _dev():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
