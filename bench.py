@@ -51,6 +51,11 @@ WORKLOADS = {
                        nt=3000, order=8, abcn=50, nshots=32, f0=8.0, seed=2, per_launch=32),
     "acoustic3d": dict(name="acoustic3d_256x512x512_nt2000_8shots_bs", shape=(256, 512, 512), dh=10.0,
                        dt=1e-3, nt=2000, order=8, abcn=50, nshots=8, f0=8.0, seed=4, per_launch=4),
+    # the reference's CPU-runnable case: two-layer vp (1500 over 2500 m/s), one source at (x 32, z 48), one
+    # receiver at (x 96, z 48), gradient of 0.5 ||pred - obs||^2 from a constant 1500 m/s start
+    "acoustic2d_small": dict(name="acoustic2d_96x128_nt800_1shot_bs", shape=(96, 128), dh=10.0, dt=2e-3, nt=800,
+                             order=8, abcn=50, nshots=1, f0=10.0, delay=0.14, seed=0, per_launch=1,
+                             two_layer=True, src=(32, 48), rec=(96, 48)),
     "elastic2d": dict(name="elastic2d_512x1024_nt4000_16shots_bs", shape=(512, 1024), dh=5.0, dt=5e-4,
                       nt=4000, order=8, abcn=50, nshots=16, f0=12.0, seed=6, per_launch=16, nrec=400),
 }
@@ -82,6 +87,15 @@ def memory_strategy(spec, elastic):
     raise SystemExit(f"unknown --memory {spec!r}")
 
 
+def two_layer_case(W):
+    """(vp_true, vp_init, sources (1, 2), receivers (1, 1, 2)) of the two-layer workload."""
+    nz, nx = W["shape"]
+    vp_true = np.full((nz, nx), 1500.0, np.float32)
+    vp_true[nz // 2:] = 2500.0
+    return (vp_true, np.full((nz, nx), 1500.0, np.float32), np.array([W["src"]], np.int64),
+            np.array([[W["rec"]]], np.int64))
+
+
 def make_problem(W, dev, nt, memory=None):
     """(solver, vp_true, vp_init, sources, receivers, mask or None, wavelet)."""
     from sweep_amd import synthetic
@@ -89,7 +103,12 @@ def make_problem(W, dev, nt, memory=None):
     from sweep_amd.options import BoundarySaving
     from sweep_amd.propagator import PropTorch
     t = np.arange(nt) * W["dt"]
-    wav = synthetic.ricker(t - 1.5 / W["f0"], W["f0"])
+    wav = synthetic.ricker(t - W.get("delay", 1.5 / W["f0"]), W["f0"])
+    if W.get("two_layer"):
+        vp_true, vp_init, src, rec = two_layer_case(W)
+        solver = PropTorch(Acoustic(spatial_order=W["order"], device=dev), shape=W["shape"], dh=W["dh"],
+                           dt=W["dt"], device=dev, abcn=W["abcn"], memory=memory_strategy(memory, False))
+        return solver, [vp_true], [vp_init], src, rec, None, wav
     if W["name"].startswith("elastic"):
         from sweep_amd.equations import Elastic
         from sweep_amd.options import Ckpt
@@ -183,7 +202,7 @@ def cpu_baseline(W, sample_nt):
     from oracle import oracle
     from sweep_amd import synthetic
     t = np.arange(sample_nt) * W["dt"]
-    wav = np.ascontiguousarray(synthetic.ricker(t - 1.5 / W["f0"], W["f0"])[None, None, :])
+    wav = np.ascontiguousarray(synthetic.ricker(t - W.get("delay", 1.5 / W["f0"]), W["f0"])[None, None, :])
     if W["name"].startswith("elastic"):
         P = oracle.E2DProblem(W["order"], W["shape"], W["dh"], W["dt"], sample_nt, abcn=W["abcn"], B=1)
         vp = synthetic.smoothed(synthetic.layered_2d(W["shape"], seed=W["seed"], vmin=1800.0, vmax=4500.0))
@@ -199,7 +218,12 @@ def cpu_baseline(W, sample_nt):
         return {"value": 1.0 / (dt * scale), "unit": "shot-gradients/s", "cores": cores, "kind": "port",
                 "sample": f"1 shot, {sample_nt} of {W['nt']} steps of forward + full-storage backward on the full "
                           f"{P.nz}x{P.nx} runtime grid ({dt:.1f} s), extrapolated x{scale:g}"}
-    if len(W["shape"]) == 2:
+    if W.get("two_layer"):
+        P = oracle.A2DProblem(W["order"], W["shape"], W["dh"], W["dt"], sample_nt, abcn=W["abcn"], B=1)
+        _, vp0, src, rec = two_layer_case(W)
+        vp = P.pad_model(vp0)
+        grid = f"{P.nz}x{P.nx}"
+    elif len(W["shape"]) == 2:
         P = oracle.A2DProblem(W["order"], W["shape"], W["dh"], W["dt"], sample_nt, abcn=W["abcn"], B=1)
         vp = P.pad_model(synthetic.smoothed(synthetic.layered_2d(W["shape"], seed=W["seed"])))
         src, rec, _ = synthetic.surface_acquisition(W["shape"], 1, seed=W["seed"], nrec_range=(300, 300))
