@@ -77,13 +77,11 @@ def _gpu_run(order, shape, dh, dt, abcn, mode, vp, wav, src, rec, res, dev, per_
     return out
 
 
-@pytest.mark.parametrize("case", FIXTURES + BS_FIXTURES)
-@pytest.mark.parametrize("mode", ["full", "bs", "ckpt"])
+@pytest.mark.parametrize("case,mode", [(c, m) for m in ("full", "bs", "ckpt") for c in FIXTURES] +
+                         [(c, "bs") for c in BS_FIXTURES])
 def test_matches_reference_core_fixture(case, mode):
     """Full-mode fixtures for every memory mode; the boundary-saving fixtures
     (the reference's own backward_bs_core) for the boundary-saving mode."""
-    if "_bs_" in case and mode != "bs":
-        pytest.skip("the boundary-saving fixtures pin the boundary-saving mode")
     dev = _dev()
     print(f"[acoustic3d] fixture={case} mode={mode}", flush=True)
     z = np.load(GOLDEN / f"{case}.npz")
