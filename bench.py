@@ -1,7 +1,15 @@
 """Benchmark: acoustic FWI gradients (forward + adjoint, boundary saving).
 
-Default workload, ``acoustic2d`` (one "step" = one gradient over the whole
-shot batch):
+Default workload, ``acoustic3d`` (one "step" = one launch: the gradients of
+``--shots-per-launch`` shots, default 2, taken in turn from the 8 shots):
+  grid (nz, ny, nx) = 256 x 512 x 512 at dh = 10 m, dt = 1 ms, nt = 2000,
+  8th order in space, CPML 50 cells (runtime grid 364 x 620 x 620), seeded
+  dipping-layer vp (1500-4000 m/s), 8 shots each recorded by a 64 x 64
+  surface receiver patch; loss 0.5 ||pred - obs||^2 against the record of
+  the true model, gradient w.r.t. the smoothed model by boundary saving
+  (fp32 strips: 42 GB per shot at nt = 2000).
+
+``--workload acoustic2d`` (one step = one gradient over the whole shot batch):
   grid 384 x 1152 at dh = 8 m, dt = 0.8 ms, nt = 3000, 8th order in space,
   CPML 50 cells, seeded sinuous-layer vp (1500-4500 m/s, +-5% smoothed noise),
   32 surface shots (Ricker 8 Hz, delay 1.5/f) with uneven receiver spreads
@@ -14,21 +22,18 @@ shot batch):
   50 cells, seeded layered vp (1800-4500 m/s), vs = vp / 1.8 +-10%, Gardner
   rho = 310 vp^0.25; 16 shots of a vertical-force (vz) Ricker at 12 Hz, 400
   receivers each recording vx and vz; gradients w.r.t. vp, vs and rho by
-  checkpointing (100 snapshots, chunk replay).
+  boundary saving.
 
-``--workload acoustic3d``:
-  grid (nz, ny, nx) = 256 x 512 x 512 at dh = 10 m, dt = 1 ms, nt = 2000,
-  8th order, CPML 50 cells, seeded dipping-layer vp (1500-4000 m/s), 8 shots
-  each recorded by a 64 x 64 surface receiver patch; one step = the 8 shot
-  gradients, computed ``--shots-per-launch`` shots at a time (default 4: the
-  boundary strips of one shot take 42 GB at nt = 2000, so 4 shots hold
-  ~172 GB of tape and ~66 GB of workspace).
+``--workload acoustic2d_small``: the 96 x 128 two-layer case (nt 800, one
+  shot and receiver) at order 8 with boundary saving; the reference's README
+  example runs it at its default order 4.
 
 Prints ONE JSON line: shot-gradients/s (value = shots x steps / time), the
 roofline of the dominant kernel (algorithmic bytes per launch over the mean
 launch duration, measured with HIP events on the launch stream during the
-timed steps), and the oracle's CPU time for a bounded sample of the same
-workload on this machine's host cores.
+timed steps), and the CPU baseline: the reference's own host core
+(``oracle/_ref``, built from its sources) when present, else the oracle, on
+a bounded sample of one shot on this machine's host cores.
 """
 from __future__ import annotations
 
@@ -50,9 +55,11 @@ WORKLOADS = {
     "acoustic2d": dict(name="acoustic2d_384x1152_nt3000_32shots_bs", shape=(384, 1152), dh=8.0, dt=8e-4,
                        nt=3000, order=8, abcn=50, nshots=32, f0=8.0, seed=2, per_launch=32),
     "acoustic3d": dict(name="acoustic3d_256x512x512_nt2000_8shots_bs", shape=(256, 512, 512), dh=10.0,
-                       dt=1e-3, nt=2000, order=8, abcn=50, nshots=8, f0=8.0, seed=4, per_launch=4),
-    # the reference's CPU-runnable case: two-layer vp (1500 over 2500 m/s), one source at (x 32, z 48), one
-    # receiver at (x 96, z 48), gradient of 0.5 ||pred - obs||^2 from a constant 1500 m/s start
+                       dt=1e-3, nt=2000, order=8, abcn=50, nshots=8, f0=8.0, seed=4, per_launch=2,
+                       step="launch"),
+    # the reference's CPU-runnable case (its README example, run here at order 8 where the README uses the
+    # default order 4): two-layer vp (1500 over 2500 m/s), one source at (x 32, z 48), one receiver at
+    # (x 96, z 48), gradient of 0.5 ||pred - obs||^2 from a constant 1500 m/s start
     "acoustic2d_small": dict(name="acoustic2d_96x128_nt800_1shot_bs", shape=(96, 128), dh=10.0, dt=2e-3, nt=800,
                              order=8, abcn=50, nshots=1, f0=10.0, delay=0.14, seed=0, per_launch=1,
                              two_layer=True, src=(32, 48), rec=(96, 48)),
@@ -246,6 +253,11 @@ def cpu_baseline(W, sample_nt):
                       f"on the full {grid} runtime grid ({dt:.1f} s), extrapolated x{scale:g}"}
 
 
+# Steps of one shot the reference host core runs for the CPU baseline (a
+# bounded sample of 10-30 s on the host cores, extrapolated to the job).
+REF_SAMPLE_STEPS = {"acoustic3d": 8, "acoustic2d": 1000, "elastic2d": 300, "acoustic2d_small": None}
+
+
 # ----------------------------------------------------------------- traffic
 # HBM bytes per launch of a kernel class, measured by the two rocprofv3 PMC
 # passes recorded in profiles/ (FETCH_SIZE scaled by 2.000 and WRITE_SIZE by
@@ -277,9 +289,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="acoustic2d")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="acoustic3d")
     ap.add_argument("--timing-stride", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-port", action="store_true",
+                    help="time the oracle (the C restatement) as the CPU baseline even where the reference's "
+                         "host core is built")
     ap.add_argument("--cpu-sample-nt", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None, help="override nt (profiling runs only)")
     ap.add_argument("--shots-per-launch", type=int, default=None)
@@ -337,12 +352,19 @@ def main():
     masks = [None if mask is None else torch.tensor(mask[a:b], device=dev) for a, b in batches]
     m0 = [torch.tensor(m, device=dev) for m in vp_init]
 
+    per_step_launch = W.get("step") == "launch"
+    launches = [0]
+
     def step():
+        """One gradient over all batches, or (per_step_launch) one batch in turn."""
         ms = [m.clone().requires_grad_(True) for m in m0]
-        for k, (a, b) in enumerate(batches):
+        todo = [launches[0] % len(batches)] if per_step_launch else range(len(batches))
+        for k in todo:
+            a, b = batches[k]
             pred = solver(wav, src[a:b], rec[a:b], models=ms)
             loss = l2_misfit(pred, obs[k], masks[k])
             loss.backward()
+            launches[0] += 1
         return ms[0].grad
 
     for k in range(args.warmup):
@@ -376,7 +398,8 @@ def main():
                           gbs=abytes[c] / (ms / n * 1e-3) / 1e9)
     dom = max(per, key=lambda c: per[c]["total_est_ms"])
     ach = per[dom]["gbs"]
-    value = nshots * args.steps / elapsed
+    shots_per_step = per_launch if per_step_launch else nshots
+    value = shots_per_step * args.steps / elapsed
     out = {
         "metric": "shot-gradients/s fwd+adj", "value": value, "unit": "shot-gradients/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -385,6 +408,8 @@ def main():
         "config": {"workload": W["name"] if args.nt is None else f"{W['name']}_nt{nt}",
                    "grid": list(W["shape"]), "runtime_grid": list(grid), "nt": nt, "shots": nshots,
                    "shots_per_launch": per_launch, "order": W["order"],
+                   "step": (f"one launch of {per_launch} of the {nshots} shots, taken in turn" if per_step_launch
+                            else f"one gradient over all {nshots} shots"),
                    "memory": args.memory or "bs",
                    "tape_bytes_per_launch": tape_bytes,
                    **({"autotuned": autotuned} if autotuned else {})},
@@ -398,9 +423,19 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
-        sample = args.cpu_sample_nt or (8 if len(W["shape"]) == 3 else (3000 if elastic else W["nt"]))
-        out["cpu_baseline"] = cpu_baseline(W, sample)
+        from oracle import refcore
+        if refcore.available() and not args.cpu_port:
+            # the reference's own CPU path: its host core built from its sources
+            from oracle import time_ref
+            steps = args.cpu_sample_nt or REF_SAMPLE_STEPS[args.workload]
+            print(f"[bench] timing the reference host core on {steps} steps of one shot", file=sys.stderr,
+                  flush=True)
+            ref = time_ref.time_sample(args.workload, steps, order=W["order"])
+            out["cpu_baseline"] = {k: ref[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        else:
+            print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
+            sample = args.cpu_sample_nt or (8 if len(W["shape"]) == 3 else (3000 if elastic else W["nt"]))
+            out["cpu_baseline"] = cpu_baseline(W, sample)
     print(json.dumps(out), flush=True)
 
 

@@ -152,6 +152,18 @@ SWP_CAPI void *swp_get_stream(void);
 SWP_CAPI void swp_set_timing(int stride);
 SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launches);
 
+/* Launch census (no reference counterpart: a test hook).  on = 1 clears it
+ * and turns it on, 0 turns it off.  While on, every launch of a 2-D z-march
+ * kernel (sites a2d.fwd_int, a2d.fwd_band, a2d.adj_int, a2d.adj_band,
+ * a2d.recon_zm), of a 3-D region kernel (a3d.fwd3.XYZ, a3d.adj3.XYZ with X,
+ * Y, Z = 1 where the region lies in that axis' CPML margin, a3d.recon_zm)
+ * and of the per-cell stencil kernels (a2d.fwd_cell, a2d.adj_cell,
+ * a3d.fwd_cell, a3d.adj_cell) is counted with its workgroups.  swp_census writes "site launches workgroups" lines into
+ * buf (NUL-terminated, truncated to cap) and returns the full text length,
+ * or -1. */
+SWP_CAPI void swp_census_enable(int on);
+SWP_CAPI int64_t swp_census(char *buf, int64_t cap);
+
 /* The binomial checkpoint schedule SWP_MEM_BINOMIAL runs, for inspection
  * and host-side tests: writes min(count, cap) (code, arg) int32 pairs --
  * code 0 step arg, 1 save to slot arg, 2 load slot arg, 3 re-run and reverse

@@ -75,6 +75,7 @@ EXPORTS = (
     "swp_nn_conv_fwd", "swp_nn_conv_wgrad_scratch", "swp_nn_conv_wgrad", "swp_nn_conv_flip", "swp_nn_relu_bwd",
     "swp_nn_pool2_fwd", "swp_nn_pool2_bwd_acc", "swp_nn_up2_fwd", "swp_nn_up2_bwd", "swp_nn_pack_input",
     "swp_nn_mse", "swp_nn_adam", "swp_nn_to_bf16", "swp_set_tuning", "swp_get_tuning",
+    "swp_census_enable", "swp_census",
 )
 TUNE_TZ2, TUNE_TZ2_BAND, TUNE_TZ2_RECON, TUNE_TZ3 = 0, 1, 2, 3
 
@@ -143,6 +144,10 @@ def _declare(L):
     L.swp_get_tuning.argtypes = [_i]
     L.swp_nn_conv_wgrad_scratch.restype = _i64
     L.swp_nn_conv_wgrad_scratch.argtypes = [_i, _i, _i, _i, _i, _i]
+    L.swp_census_enable.restype = None
+    L.swp_census_enable.argtypes = [_c]
+    L.swp_census.restype = _i64
+    L.swp_census.argtypes = [ctypes.c_char_p, _i64]
     L.swp_l2_misfit.restype = ctypes.c_int
     L.swp_l2_misfit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _i, _i, _i, _i,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
@@ -256,6 +261,26 @@ def get_tuning(key: int) -> int:
     if v < 0:
         raise ValueError(f"bad tuning key {key}")
     return int(v)
+
+
+def census_enable(on: bool = True):
+    """Clear the launch census and turn it on (or turn it off)."""
+    load().swp_census_enable(1 if on else 0)
+
+
+def census() -> dict:
+    """{site: (launches, workgroups)} counted since census_enable(True)."""
+    L = load()
+    n = L.swp_census(None, 0)
+    if n < 0:
+        raise RuntimeError("swp_census failed")
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    L.swp_census(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        site, launches, blocks = line.split()
+        out[site] = (int(launches), int(blocks))
+    return out
 
 
 def _pairs(fn, nt: int, slots: int):

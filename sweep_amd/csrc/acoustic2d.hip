@@ -486,6 +486,7 @@ void launch_fwd(const Ctx &x, const StepFields &f, float *u_this, float *strip)
     }
     const Geo &g = x.g;
     const dim3 grid = grid_for(g.nx - 2 * g.M, g.nz - 2 * g.M, g.B);
+    census_add("a2d.fwd_cell", int64_t(grid.x) * grid.y * grid.z);
     by_order(g.M, [&](auto ms) {
         hipLaunchKernelGGL(fwd_kernel<decltype(ms)::value>, grid, dim3(64, 4), 0, stream(), g, x.c,
                            x.pr, f, u_this, strip);
@@ -502,6 +503,7 @@ void launch_adj(const Ctx &x, const AdjFields &f, const float *img, float *grad)
     }
     const Geo &g = x.g;
     const dim3 grid = grid_for(g.nx - 2 * g.M, g.nz - 2 * g.M, g.B);
+    census_add("a2d.adj_cell", int64_t(grid.x) * grid.y * grid.z);
     by_order(g.M, [&](auto ms) {
         hipLaunchKernelGGL(adj_kernel<decltype(ms)::value>, grid, dim3(64, 4), 0, stream(), g, x.c,
                            x.pr, f, img, grad);

@@ -653,6 +653,7 @@ void launch_fwd_zm(const Ctx &x, const ZmPlan &p, const StepFields &f, float *u_
         constexpr int M = decltype(mc)::value;
         if (p.nband) {
             const dim3 grid(unsigned(p.nband), 1, unsigned(g.B));
+            census_add("a2d.fwd_band", int64_t(p.nband) * g.B);
             if (u_this) {
                 if (strip) hipLaunchKernelGGL((fwd_band_kernel<M, true, true>), grid, dim3(64), 0, stream(), g, x.c, x.pr, f, u_this, strip, p.band);
                 else hipLaunchKernelGGL((fwd_band_kernel<M, true, false>), grid, dim3(64), 0, stream(), g, x.c, x.pr, f, u_this, strip, p.band);
@@ -663,6 +664,7 @@ void launch_fwd_zm(const Ctx &x, const ZmPlan &p, const StepFields &f, float *u_
         }
         if (p.ninner) {
             const dim3 grid(unsigned(p.ninner), 1, unsigned(g.B));
+            census_add("a2d.fwd_int", int64_t(p.ninner) * g.B);
             if (u_this) hipLaunchKernelGGL((fwd_int_kernel<M, true>), grid, dim3(64), 0, stream(), g, x.c, f, u_this, p.inner);
             else hipLaunchKernelGGL((fwd_int_kernel<M, false>), grid, dim3(64), 0, stream(), g, x.c, f, u_this, p.inner);
         }
@@ -678,11 +680,13 @@ void launch_adj_zm(const Ctx &x, const ZmPlan &p, const AdjFields &f, const floa
         constexpr int M = decltype(mc)::value;
         if (p.nband) {
             const dim3 grid(unsigned(p.nband), 1, unsigned(g.B));
+            census_add("a2d.adj_band", int64_t(p.nband) * g.B);
             if (im) hipLaunchKernelGGL((adj_band_kernel<M, true>), grid, dim3(64), 0, stream(), g, x.c, x.pr, f, img, grad, p.band);
             else hipLaunchKernelGGL((adj_band_kernel<M, false>), grid, dim3(64), 0, stream(), g, x.c, x.pr, f, img, grad, p.band);
         }
         if (p.ninner) {
             const dim3 grid(unsigned(p.ninner), 1, unsigned(g.B));
+            census_add("a2d.adj_int", int64_t(p.ninner) * g.B);
             if (im) hipLaunchKernelGGL((adj_int_kernel<M, true>), grid, dim3(64), 0, stream(), g, x.c, f, img, grad, p.inner);
             else hipLaunchKernelGGL((adj_int_kernel<M, false>), grid, dim3(64), 0, stream(), g, x.c, f, img, grad, p.inner);
         }
@@ -696,6 +700,7 @@ void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const f
     const Geo &g = x.g;
     const dim3 grid(unsigned((x_hi - (x_lo - x_lo % 32) + 63) / 64), unsigned((z_hi - z_lo + tz - 1) / tz),
                     unsigned(g.B));
+    census_add("a2d.recon_zm", int64_t(grid.x) * grid.y * grid.z);
     by_m(g.M, [&](auto mc) {
         constexpr int M = decltype(mc)::value;
         if (grad) hipLaunchKernelGGL((recon_zm_kernel<M, true>), grid, dim3(64), 0, stream(), g, x.c, vp, u_prev, u_now, u_next, adj, grad, x_lo, x_hi, z_lo, z_hi, tz);

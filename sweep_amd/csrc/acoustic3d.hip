@@ -418,6 +418,7 @@ static void launch_fwd(const Ctx &x, const StepFields &f, float *u_this, float *
     if (x.zm && !x.shell_percell) return;
     by_order(g.M, [&](auto ms) {
         per_box(g, x.zm, [&](const Box &bx, dim3 grid) {
+            census_add("a3d.fwd_cell", int64_t(grid.x) * grid.y * grid.z);
             hipLaunchKernelGGL(fwd_kernel<decltype(ms)::value>, grid, dim3(64, 4), 0, stream(), g, x.c, x.pr, f,
                                u_this, strip, bx);
         });
@@ -433,6 +434,7 @@ static void launch_adj(const Ctx &x, const AdjFields &f, const float *img, float
     if (x.zm && !x.shell_percell) return;
     by_order(g.M, [&](auto ms) {
         per_box(g, x.zm, [&](const Box &bx, dim3 grid) {
+            census_add("a3d.adj_cell", int64_t(grid.x) * grid.y * grid.z);
             hipLaunchKernelGGL(adj_kernel<decltype(ms)::value>, grid, dim3(64, 4), 0, stream(), g, x.c, x.pr, f,
                                img, grad, bx);
         });

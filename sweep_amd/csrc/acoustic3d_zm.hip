@@ -1108,6 +1108,10 @@ void fwd_region(const Ctx &x, const StepFields &f, float *u_this, float *strip)
     constexpr int RY = SHELL ? ry_fwd<BX, BY, BZ>() : kRyFwd;
     const ZSet s = region_set(x.g, BX, BY, BZ, x.tz, 4 * RY);
     if (!s.per_shot) return;
+    if (census_on()) {
+        const char site[] = {'a', '3', 'd', '.', 'f', 'w', 'd', '3', '.', char('0' + BX), char('0' + BY), char('0' + BZ), 0};
+        census_add(site, int64_t(s.per_shot) * x.g.B);
+    }
     hipLaunchKernelGGL((fwd3_kernel<M, RY, BX, BY, BZ>), grid_of(s, x.g.B), dim3(256), 0, stream(), x.g, x.c, x.pr,
                        f, u_this, strip, s);
 }
@@ -1119,6 +1123,10 @@ void adj_region(const Ctx &x, const AdjFields &f, const float *img, float *grad)
     constexpr int RY = SHELL ? ry_adj<BX, BY, BZ>() : kRyAdj;
     const ZSet s = region_set(x.g, BX, BY, BZ, x.tz, 4 * RY);
     if (!s.per_shot) return;
+    if (census_on()) {
+        const char site[] = {'a', '3', 'd', '.', 'a', 'd', 'j', '3', '.', char('0' + BX), char('0' + BY), char('0' + BZ), 0};
+        census_add(site, int64_t(s.per_shot) * x.g.B);
+    }
     hipLaunchKernelGGL((adj3_kernel<M, RY, BX, BY, BZ>), grid_of(s, x.g.B), dim3(256), 0, stream(), x.g, x.c, x.pr,
                        f, img, grad, s);
 }
@@ -1193,6 +1201,7 @@ void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const f
     ZSet s{};
     add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyRecon, x.g.px);
     if (!s.per_shot) return;
+    census_add("a3d.recon_zm", int64_t(s.per_shot) * g.B);
     by_order(g.M, [&](auto ms) {
         constexpr int M = decltype(ms)::value;
         hipLaunchKernelGGL((recon_zm_kernel<M, kRyRecon>), grid_of(s, g.B), dim3(256), 0, stream(), g, x.c, vp, u_prev,

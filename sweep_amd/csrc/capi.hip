@@ -8,6 +8,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "acoustic2d.h"
@@ -22,6 +24,10 @@ namespace swp {
 namespace {
 thread_local hipStream_t t_stream = nullptr;
 std::atomic<int> g_tuning[SWP_TUNE_COUNT];
+
+std::atomic<bool> g_census_on{false};
+std::mutex g_census_mu;
+std::map<std::string, std::pair<int64_t, int64_t>> g_census;   // site -> (launches, workgroups)
 
 struct Entry {
     const char *name;
@@ -101,6 +107,17 @@ int tuning(int key, const char *env, int def)
 }
 
 bool eq_available(int eq) { return eq == 0 || (eq == 1 && a3d::available()) || (eq == 2 && e2d::available()); }
+
+bool census_on() { return g_census_on.load(std::memory_order_relaxed); }
+
+void census_add(const char *site, int64_t blocks)
+{
+    if (!census_on()) return;
+    std::lock_guard<std::mutex> lk(g_census_mu);
+    auto &e = g_census[site];
+    e.first += 1;
+    e.second += blocks;
+}
 
 }  // namespace swp
 
@@ -243,6 +260,33 @@ SWP_CAPI int swp_last_kernel_stats(int which, double *ms_total, int64_t *launche
         if (ms_total) *ms_total = ms;
         if (launches) *launches = n;
     });
+}
+
+SWP_CAPI void swp_census_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(g_census_mu);
+    if (on) g_census.clear();
+    g_census_on.store(on != 0);
+}
+
+SWP_CAPI int64_t swp_census(char *buf, int64_t cap)
+{
+    try {
+        std::string s;
+        {
+            std::lock_guard<std::mutex> lk(g_census_mu);
+            for (const auto &kv : g_census)
+                s += kv.first + " " + std::to_string(kv.second.first) + " " + std::to_string(kv.second.second) + "\n";
+        }
+        if (buf && cap > 0) {
+            const size_t n = std::min<size_t>(s.size(), size_t(cap - 1));
+            std::memcpy(buf, s.data(), n);
+            buf[n] = '\0';
+        }
+        return int64_t(s.size());
+    } catch (...) {
+        return -1;
+    }
 }
 
 SWP_CAPI int64_t swp_revolve_schedule(int32_t nt, int32_t slots, int32_t *ops, int64_t cap)

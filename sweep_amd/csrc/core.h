@@ -93,4 +93,10 @@ struct TimedScope {
 // Zero a float buffer with a fill kernel (graph-capture safe: no memset node).
 void fill_zero(float *p, int64_t n);
 
+// Launch census: while it is on (swp_census_enable), every z-march and
+// region launch adds one launch and its workgroup count under its site name,
+// so a test can assert which kernel variants a case actually ran.
+bool census_on();
+void census_add(const char *site, int64_t blocks);
+
 }  // namespace swp

@@ -1,0 +1,324 @@
+"""GPU parity of the z-marching kernels on grids where their lean interior
+variants run, and property checks at the full 2-D and 3-D workload grids.
+
+The smaller parity grids elsewhere leave the interior sets of the z-march
+plans empty: the 2-D interior kernels need a whole 64-column strip inside
+[pad + 2M + 1, nx - pad - 2M - 1), the 3-D x-interior starts at the next
+multiple of 32 after pad + 2M + 1.  The grids here are sized so that
+
+* 2-D (96 x 256, abcn 20, runtime 144 x 304 at order 8): three interior
+  strips of 64 columns and two 32-row interior segments;
+* 3-D (40 x 36 x 96, abcn 10, runtime 68 x 64 x 124 at order 8): every axis
+  has a low margin, an interior and a high margin, so all eight region
+  kernels (interior, three faces, three edges, corners) get blocks;
+
+and every test asserts, through the core's launch census, that the
+interior and region kernels it is meant to cover ran.  Bars: records
+rel-L2 <= 1e-5, model and wavelet gradients <= 1e-4 against the oracle on
+the same inputs (3e-4 for bf16 strips, where one-ulp differences of a strip
+value can round to the neighbouring bfloat16).
+
+The full-grid checks run one shot on the 384 x 1152 and 256 x 512 x 512
+workload grids at reduced nt: full storage, boundary saving and binomial
+checkpointing give the same gradients, the adjoint identity
+<F w, r> = <w, F^T r> holds, and (2-D) the boundary-saving gradient of the
+32-shot uneven acquisition matches central differences at 20 random cells.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+@pytest.fixture
+def zmarch(monkeypatch):
+    monkeypatch.setenv("SWP_KERNELS", "zmarch")
+    yield
+    from sweep_amd import _core
+    _core.census_enable(False)
+
+
+def _memory(name):
+    from sweep_amd.options import BoundarySaving, Ckpt, Full
+    if name.startswith("binomial"):
+        return Ckpt(mode="recursive", count=int(name[len("binomial"):]))
+    return {"full": Full(), "bs": BoundarySaving(), "bs_bf16": BoundarySaving(storage_dtype="bf16")}[name]
+
+
+def _canon(raw):
+    return np.ascontiguousarray(np.asarray(raw).transpose(0, 2, 1)[..., None])
+
+
+def _run(equation, shape, dh, dt, abcn, mode, vp, wav, src, rec, res, dev):
+    """Forward + backward of sum(pred * res) through PropTorch; returns the
+    raw record, the model and wavelet gradients and the launch census."""
+    from sweep_amd import _core
+    from sweep_amd.propagator import PropTorch
+    solver = PropTorch(equation, shape=tuple(shape), dh=dh, dt=dt, device=dev, abcn=abcn, memory=_memory(mode))
+    m = torch.tensor(vp, device=dev, requires_grad=True)
+    w = torch.tensor(wav, device=dev, requires_grad=True)
+    _core.census_enable(True)
+    pred = solver(w, src, rec, models=[m])
+    (pred * torch.tensor(_canon(res), device=dev)).sum().backward()
+    torch.cuda.synchronize()
+    census = _core.census()
+    _core.census_enable(False)
+    return dict(rec=pred.detach().cpu().numpy()[..., 0].transpose(0, 2, 1), grad_vp=m.grad.cpu().numpy(),
+                grad_wavelet=w.grad.cpu().numpy(), census=census)
+
+
+# ---------------------------------------------------------------- 2-D
+def _case2d(order, seed, B=3, nt=300, shape=(96, 256), abcn=20):
+    from sweep_amd import synthetic
+    rng = np.random.default_rng(seed)
+    vp = synthetic.layered_2d(shape, seed=seed, vmin=1500.0, vmax=3000.0)
+    dh, dt = 8.0, 8e-4
+    src, rec, _ = synthetic.surface_acquisition(shape, B, seed=seed, nrec_range=(40, 90), depth=30)
+    wav = np.stack([synthetic.ricker(np.arange(nt) * dt - 0.1, 15.0) * (1.0 + 0.2 * b) for b in range(B)])
+    res = rng.standard_normal((B, rec.shape[1], nt)).astype(np.float32)
+    return dict(order=order, shape=shape, dh=dh, dt=dt, abcn=abcn, nt=nt, B=B, vp=vp, wav=wav.astype(np.float32),
+                src=src, rec=rec, res=res)
+
+
+def _oracle2d(c, mode):
+    from oracle.oracle import A2DProblem, lib
+    P = A2DProblem(c["order"], c["shape"], c["dh"], c["dt"], c["nt"], abcn=c["abcn"], B=c["B"])
+    lib().orc_set_strip_bf16(1 if mode == "bs_bf16" else 0)
+    try:
+        o = P.gradient({"full": 0, "bs": 1, "bs_bf16": 1}[mode], P.pad_model(c["vp"]), c["wav"][:, None, :],
+                       P.shift(c["src"][:, None, :]), P.shift(c["rec"]), c["res"])
+    finally:
+        lib().orc_set_strip_bf16(0)
+    return dict(rec=o["rec"], grad_vp=P.crop(o["grad_vp"]).sum(0), grad_wavelet=o["grad_wavelet"][:, 0])
+
+
+@pytest.mark.parametrize("order", [8, 4])
+@pytest.mark.parametrize("mode", ["full", "bs", "bs_bf16"])
+def test_acoustic2d_interior_strips_match_oracle(order, mode, zmarch):
+    dev = _dev()
+    from sweep_amd.equations import Acoustic
+    print(f"[zmarch2d] order={order} mode={mode} grid=96x256 abcn=20 B=3 nt=300", flush=True)
+    c = _case2d(order, seed=40 + order)
+    got = _run(Acoustic(spatial_order=order, device=dev), c["shape"], c["dh"], c["dt"], c["abcn"], mode, c["vp"],
+               c["wav"], c["src"], c["rec"], c["res"], dev)
+    cen = got["census"]
+    print(f"[zmarch2d] census {cen}", flush=True)
+    for site in ("a2d.fwd_int", "a2d.fwd_band", "a2d.adj_int", "a2d.adj_band"):
+        assert cen.get(site, (0, 0))[1] > 0, (site, cen)
+    if mode != "full":
+        assert cen.get("a2d.recon_zm", (0, 0))[1] > 0, cen
+    assert "a2d.fwd_cell" not in cen and "a2d.adj_cell" not in cen, cen
+    ref = _oracle2d(c, mode)
+    errs = {k: rel_l2(got[k], ref[k]) for k in ("rec", "grad_vp", "grad_wavelet")}
+    print(f"[zmarch2d] order {order} {mode} {errs}", flush=True)
+    gbar = 3e-4 if mode == "bs_bf16" else 1e-4
+    assert errs["rec"] <= 1e-5
+    assert errs["grad_vp"] <= gbar
+    assert errs["grad_wavelet"] <= gbar
+
+
+def test_acoustic2d_interior_grid_per_cell_family(monkeypatch):
+    """The same grid on the per-cell kernels (the automatic choice at this
+    size) agrees with the oracle too."""
+    dev = _dev()
+    monkeypatch.delenv("SWP_KERNELS", raising=False)
+    from sweep_amd.equations import Acoustic
+    c = _case2d(8, seed=48)
+    got = _run(Acoustic(spatial_order=8, device=dev), c["shape"], c["dh"], c["dt"], c["abcn"], "bs", c["vp"],
+               c["wav"], c["src"], c["rec"], c["res"], dev)
+    assert got["census"].get("a2d.fwd_cell", (0, 0))[1] > 0, got["census"]
+    ref = _oracle2d(c, "bs")
+    errs = {k: rel_l2(got[k], ref[k]) for k in ("rec", "grad_vp", "grad_wavelet")}
+    print(f"[percell2d] {errs}", flush=True)
+    assert errs["rec"] <= 1e-5
+    assert errs["grad_vp"] <= 1e-4
+    assert errs["grad_wavelet"] <= 1e-4
+
+
+# ---------------------------------------------------------------- 3-D
+REGIONS = [f"{x}{y}{z}" for x in "01" for y in "01" for z in "01"]
+
+
+def _case3d(order, seed, B=2, nt=120, shape=(40, 36, 96), abcn=10):
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    v = gaussian_filter(rng.standard_normal(shape), 2.0)
+    v = (v - v.min()) / (v.max() - v.min())
+    vp = (1800.0 + 900.0 * v).astype(np.float32)
+    dh, dt = 10.0, 1e-3
+    nz, ny, nx = shape
+    src = np.stack([rng.integers(20, nx - 20, B), rng.integers(8, ny - 8, B), rng.integers(10, nz - 10, B)], -1)
+    nrec = 40
+    rec = np.stack([rng.integers(0, nx, (B, nrec)), rng.integers(0, ny, (B, nrec)), rng.integers(0, nz, (B, nrec))],
+                   -1)
+    t = np.arange(nt) * dt - 0.06
+    a = (np.pi * 18.0 * t) ** 2
+    w0 = ((1 - 2 * a) * np.exp(-a)).astype(np.float32)
+    wav = np.stack([w0 * (1.0 + 0.1 * b) for b in range(B)]).astype(np.float32)
+    res = rng.standard_normal((B, nrec, nt)).astype(np.float32)
+    return dict(order=order, shape=shape, dh=dh, dt=dt, abcn=abcn, nt=nt, B=B, vp=vp, wav=wav, src=src, rec=rec,
+                res=res)
+
+
+def _oracle3d(c, mode):
+    from oracle.oracle import A3DProblem, lib
+    P = A3DProblem(c["order"], c["shape"], c["dh"], c["dt"], c["nt"], abcn=c["abcn"], B=c["B"])
+    lib().orc_set_strip_bf16(1 if mode == "bs_bf16" else 0)
+    try:
+        o = P.gradient({"full": 0, "bs": 1, "bs_bf16": 1}[mode], P.pad_model(c["vp"]), c["wav"][:, None, :],
+                       P.shift(c["src"][:, None, :]), P.shift(c["rec"]), c["res"])
+    finally:
+        lib().orc_set_strip_bf16(0)
+    return dict(rec=o["rec"], grad_vp=P.crop(o["grad_vp"]).sum(0), grad_wavelet=o["grad_wavelet"][:, 0])
+
+
+@pytest.mark.parametrize("order", [8, 4])
+@pytest.mark.parametrize("mode", ["full", "bs", "bs_bf16"])
+def test_acoustic3d_all_regions_match_oracle(order, mode, zmarch):
+    dev = _dev()
+    from sweep_amd.equations import Acoustic3D
+    print(f"[zmarch3d] order={order} mode={mode} grid=40x36x96 abcn=10 B=2 nt=120", flush=True)
+    c = _case3d(order, seed=60 + order)
+    got = _run(Acoustic3D(spatial_order=order, device=dev), c["shape"], c["dh"], c["dt"], c["abcn"], mode, c["vp"],
+               c["wav"], c["src"], c["rec"], c["res"], dev)
+    cen = got["census"]
+    print(f"[zmarch3d] census {cen}", flush=True)
+    for r in REGIONS:
+        assert cen.get(f"a3d.fwd3.{r}", (0, 0))[1] > 0, (r, cen)
+        assert cen.get(f"a3d.adj3.{r}", (0, 0))[1] > 0, (r, cen)
+    if mode != "full":
+        assert cen.get("a3d.recon_zm", (0, 0))[1] > 0, cen
+    assert "a3d.fwd_cell" not in cen and "a3d.adj_cell" not in cen, cen
+    ref = _oracle3d(c, mode)
+    errs = {k: rel_l2(got[k], ref[k]) for k in ("rec", "grad_vp", "grad_wavelet")}
+    print(f"[zmarch3d] order {order} {mode} {errs}", flush=True)
+    gbar = 3e-4 if mode == "bs_bf16" else 1e-4
+    assert errs["rec"] <= 1e-5
+    assert errs["grad_vp"] <= gbar
+    assert errs["grad_wavelet"] <= gbar
+
+
+# ---------------------------------------------------------------- full grids
+def _dot_identity(got, wav, res):
+    """<F w, r> against <w, F^T r> in float64."""
+    lhs = float(np.sum(got["rec"].astype(np.float64) * res))
+    rhs = float(np.sum(np.asarray(wav, np.float64) * got["grad_wavelet"]))
+    return abs(lhs - rhs) / max(abs(lhs), 1e-300), lhs, rhs
+
+
+def test_acoustic3d_full_grid_modes_agree():
+    """One shot on the 256 x 512 x 512 workload grid (runtime 364 x 620 x 620,
+    order 8, the default kernel choice: the z-march at this size), nt 150:
+    boundary saving and binomial checkpointing (4 slots) against full storage,
+    and the adjoint identity."""
+    dev = _dev()
+    from sweep_amd import _core, synthetic
+    from sweep_amd.equations import Acoustic3D
+    shape, nt, dh, dt = (256, 512, 512), 150, 10.0, 1e-3
+    print(f"[c4grid] shape={shape} nt={nt} one shot", flush=True)
+    vp = synthetic.dipping_3d(shape, seed=3)
+    # the workload's acquisition: a surface source inside its 64 x 64 receiver
+    # patch, so the record and the adjoint field overlap the forward field
+    # within nt (a deep source's direct wave would not reach the receivers,
+    # leaving a gradient made of rounding noise only)
+    src, rec = synthetic.patch_acquisition_3d(shape, 1, patch=64)
+    wav = synthetic.ricker(np.arange(nt) * dt - 0.06, 15.0)[None]
+    rng = np.random.default_rng(4)
+    res = rng.standard_normal((1, rec.shape[1], nt)).astype(np.float32)
+    eq = Acoustic3D(spatial_order=8, device=dev)
+    out = {}
+    for mode in ("full", "bs", "binomial4"):
+        out[mode] = _run(eq, shape, dh, dt, 50, mode, vp, wav, src, rec, res, dev)
+        print(f"[c4grid] {mode} census {out[mode]['census']}", flush=True)
+        torch.cuda.empty_cache()
+    assert out["full"]["census"].get("a3d.adj3.000", (0, 0))[1] > 0
+    for k in ("rec", "grad_vp", "grad_wavelet"):
+        assert np.array_equal(out["full"][k], out["binomial4"][k]), k
+    assert np.array_equal(out["full"]["rec"], out["bs"]["rec"])
+    assert np.array_equal(out["full"]["grad_wavelet"], out["bs"]["grad_wavelet"])
+    e = rel_l2(out["bs"]["grad_vp"], out["full"]["grad_vp"])
+    d, lhs, rhs = _dot_identity(out["full"], wav, res)
+    print(f"[c4grid] bs vs full grad_vp {e:.2e}; <Fw,r> {lhs:.9e} <w,F^T r> {rhs:.9e} rel {d:.2e}", flush=True)
+    assert np.abs(out["full"]["grad_vp"]).max() > 0
+    assert e <= 1e-4
+    assert d <= 2e-5
+    _core.census_enable(False)
+
+
+def test_acoustic2d_full_grid_modes_agree(zmarch):
+    """One shot on the 384 x 1152 workload grid (runtime 492 x 1260, order 8)
+    on the z-march, nt 1500: boundary saving and binomial checkpointing
+    against full storage, and the adjoint identity."""
+    dev = _dev()
+    from sweep_amd import synthetic
+    from sweep_amd.equations import Acoustic
+    shape, nt, dh, dt = (384, 1152), 1500, 8.0, 8e-4
+    vp = synthetic.layered_2d(shape, seed=11)
+    src, rec, _ = synthetic.surface_acquisition(shape, 1, seed=11)
+    wav = synthetic.ricker(np.arange(nt) * dt - 1.5 / 8.0, 8.0)[None]
+    res = np.random.default_rng(12).standard_normal((1, rec.shape[1], nt)).astype(np.float32)
+    eq = Acoustic(spatial_order=8, device=dev)
+    out = {mode: _run(eq, shape, dh, dt, 50, mode, vp, wav, src, rec, res, dev) for mode in ("full", "bs", "binomial6")}
+    assert out["full"]["census"].get("a2d.adj_int", (0, 0))[1] > 0
+    for k in ("rec", "grad_vp", "grad_wavelet"):
+        assert np.array_equal(out["full"][k], out["binomial6"][k]), k
+    assert np.array_equal(out["full"]["rec"], out["bs"]["rec"])
+    e = rel_l2(out["bs"]["grad_vp"], out["full"]["grad_vp"])
+    d, lhs, rhs = _dot_identity(out["full"], wav, res)
+    print(f"[c2grid] bs vs full grad_vp {e:.2e}; <Fw,r> {lhs:.9e} <w,F^T r> {rhs:.9e} rel {d:.2e}", flush=True)
+    assert e <= 1e-4
+    assert d <= 2e-5
+
+
+def test_acoustic2d_workload_gradient_fd():
+    """The 2-D workload's own acceptance check: the boundary-saving gradient
+    of 32 surface shots with uneven receiver spreads (150-500) on the
+    384 x 1152 grid at nt 3000, against central differences at 20 random
+    cells.  J(m) = <record(m), r> for a fixed random r with the padded
+    receivers masked out, accumulated in float64; each perturbation is a
+    Gaussian bump (sigma 2 cells, 1% of the local model), compared with
+    <grad, bump>.  Bars as in test_gpu_fd.py: correlation >= 0.99995, median
+    relative error <= 1%, every error <= 2% of the rms derivative."""
+    dev = _dev()
+    from sweep_amd import synthetic
+    from sweep_amd.equations import Acoustic
+    from sweep_amd.options import BoundarySaving
+    from sweep_amd.propagator import PropTorch
+    from test_gpu_fd import _bump, _check
+    shape, nt, dh, dt = (384, 1152), 3000, 8.0, 8e-4
+    vp = synthetic.layered_2d(shape, seed=0)
+    src, rec, mask = synthetic.surface_acquisition(shape, 32, seed=0)
+    wav = synthetic.ricker(np.arange(nt) * dt - 1.5 / 8.0, 8.0)
+    solver = PropTorch(Acoustic(spatial_order=8, device=dev), shape=shape, dh=dh, dt=dt, device=dev, abcn=50,
+                       memory=BoundarySaving())
+    rng = np.random.default_rng(20)
+    r = rng.standard_normal((32, nt, rec.shape[1], 1)).astype(np.float32) * mask[:, None, :, None]
+    r = torch.tensor(r, device=dev)
+    m = torch.tensor(vp, device=dev, requires_grad=True)
+    (solver(wav, src, rec, models=[m]) * r).sum().backward()
+    g = m.grad.cpu().numpy().astype(np.float64)
+    rr = r.double()
+    cells = [(int(rng.integers(12, shape[0] - 6)), int(rng.integers(6, shape[1] - 6))) for _ in range(20)]
+    fd, an = [], []
+    with torch.no_grad():
+        for k, (iz, ix) in enumerate(cells):
+            dm = 0.01 * float(vp[iz, ix]) * _bump(shape, iz, ix)
+            vals = []
+            for s in (1.0, -1.0):
+                mp = torch.tensor((vp + s * dm).astype(np.float32), device=dev)
+                vals.append(float((solver(wav, src, rec, models=[mp]).double() * rr).sum()))
+            fd.append((vals[0] - vals[1]) / 2)
+            an.append(float(np.sum(g * dm)))
+            print(f"[c2fd] cell {k} ({iz},{ix}) fd {fd[-1]:.6e} grad {an[-1]:.6e}", flush=True)
+    _check(fd, an)
