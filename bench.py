@@ -304,6 +304,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="pick shots per launch and the z-march segment heights by timing short probes first "
                          "(untimed); the choice is reported under config.autotuned")
+    ap.add_argument("--graphs", choices=["on", "off"], default="on",
+                    help="replay each launch's time loop from a captured HIP graph (on) or launch step by step")
     ap.add_argument("--pmc-launch", action="store_true",
                     help="counter runs: a 1 GiB device copy (the calibration of FETCH_SIZE / WRITE_SIZE), "
                          "then one untimed gradient launch; prints no JSON")
@@ -316,6 +318,7 @@ def main():
     from sweep_amd import _core
     from sweep_amd.loss import l2_misfit
     W = WORKLOADS[args.workload]
+    _core.set_graphs(1 if args.graphs == "on" else 0)
     nt = args.nt or W["nt"]
     per_launch = args.shots_per_launch or W["per_launch"]
     dev = torch.device("cuda:0")
@@ -371,7 +374,6 @@ def main():
         step()
         torch.cuda.synchronize()
         print(f"[bench] warmup {k + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    _core.set_timing(args.timing_stride)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -379,9 +381,17 @@ def main():
         print(f"[bench] step {k + 1}/{args.steps} launched", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    graphs = _core.graph_stats()
+    assert torch.isfinite(g).all(), "non-finite gradient"
+    # kernel durations: one more step, launched eagerly with HIP events around
+    # one launch in every timing_stride of each kernel class (a launch's
+    # duration does not depend on how it was enqueued)
+    _core.set_timing(args.timing_stride)
+    step()
+    torch.cuda.synchronize()
     stats = {c: _core.kernel_stats(c) for c in range(4)}
     _core.set_timing(0)
-    assert torch.isfinite(g).all(), "non-finite gradient"
+    print(f"[bench] kernel timing step done; graphs {graphs}", file=sys.stderr, flush=True)
 
     P = solver._last_call.problem
     grid = (P.nz, P.nx) if P.ndim == 2 else (P.nz, P.ny, P.nx)
@@ -412,6 +422,7 @@ def main():
                             else f"one gradient over all {nshots} shots"),
                    "memory": args.memory or "bs",
                    "tape_bytes_per_launch": tape_bytes,
+                   "graphs": args.graphs,
                    **({"autotuned": autotuned} if autotuned else {})},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,

@@ -47,6 +47,9 @@ enum {
     SWP_FLAG_BF16_STRIPS = 2,    /* SWP_MEM_BOUNDARY strips in bfloat16, round to nearest even
                                     (reference: BoundarySaving(storage_dtype='bf16'),
                                     propagator/options.py:204-212, boundarysaver.cu:29) */
+    SWP_FLAG_GRAPH = 4,          /* the caller keeps this call's buffers at fixed addresses call after
+                                    call: capture the call's launches into a graph and replay it
+                                    (swp_set_graphs) */
 };
 
 /* One propagation problem on the runtime grid.  Arrays are row-major with x
@@ -142,6 +145,26 @@ SWP_CAPI int swp_call(int entry, const swp_problem *in, void *io, char *err, int
  * thread-local hipStream_t every launch of this thread goes on. */
 SWP_CAPI void swp_set_stream(void *stream);
 SWP_CAPI void *swp_get_stream(void);
+
+/* Time-loop graphs (no reference counterpart: the reference launches every
+ * step from its host loop, csrc/cuda/common/eq_driver.cuh:377-434, 842-861).
+ * For a call whose problem carries SWP_FLAG_GRAPH, swp_call captures the
+ * call's whole launch sequence -- every kernel of every time step -- into a
+ * HIP graph on a side stream, and launches the graph on the thread's stream;
+ * a later call with the same entry, device, problem (buffer addresses,
+ * scalars, host coefficient values), io pointers, tuning and kernel-family
+ * settings replays it instead of launching step by step.  Results are
+ * bit-identical either way.  A shape whose buffers move between calls falls
+ * back to eager launches after three captures without a replay.  Calls
+ * without the flag, or with kernel timing or the launch census on, run
+ * eagerly.
+ * swp_set_graphs: 1 on, 0 off, -1 the SWP_GRAPHS environment variable
+ * (default on, "0" turns it off).  swp_graph_stats reports captures, replays,
+ * eager fallbacks and cached graphs since the last swp_graph_clear, which
+ * also frees every cached graph. */
+SWP_CAPI void swp_set_graphs(int mode);
+SWP_CAPI int swp_graph_stats(int64_t *captures, int64_t *replays, int64_t *eager, int64_t *cached);
+SWP_CAPI void swp_graph_clear(void);
 
 /* Kernel-class timing with HIP events on the launch stream.  stride > 0 turns
  * it on (and clears earlier samples), timing one launch in every `stride` of

@@ -14,8 +14,8 @@ workload: forward + boundary-saving backward of a seeded residual.
   and shot count (stated in the output).
 
 ``python -m oracle.time_ref [--workloads ...] [--steps K]`` prints one JSON
-line per workload (ms per forward+backward step, shot-gradients/s, the
-extrapolation, OMP_NUM_THREADS).  ``bench.py`` imports :func:`time_sample`
+line per workload (ms per forward+backward step and per forward step,
+shot-gradients/s, the extrapolation, OMP_NUM_THREADS).  ``bench.py`` imports :func:`time_sample`
 for its ``cpu_baseline`` leg (kind "reference").
 """
 from __future__ import annotations
@@ -80,16 +80,27 @@ def _inputs(name, nt, order=None):
     return P, models, np.ascontiguousarray(wav, np.float32), P.shift(src), P.shift(rec), res
 
 
-def _run_once(name, nt, order=None):
+def _run_once(name, nt, order=None, backward=True):
+    """Seconds of one reference run: forward, plus the boundary-saving
+    backward of a seeded residual when ``backward``."""
     from oracle import refcore
     P, models, wav, sl, rl, res = _inputs(name, nt, order)
+    res = res if backward else None
     t0 = time.perf_counter()
     if WORKLOADS[name][0] == "elastic":
         refcore.run_elastic((P.nz, P.nx), P.M, nt, P.abcn, P.dt, P.dh_zyx, P.prof, *models, P.grad, wav, sl, rl,
-                            res, bs=True)
+                            res, bs=backward)
     else:
-        refcore.run(P, models[0], wav, sl, rl, res, bs=True)
+        refcore.run(P, models[0], wav, sl, rl, res, bs=backward)
     return time.perf_counter() - t0, P
+
+
+def time_forward(name, steps=None, order=None):
+    """ms per forward step of the reference host core on one shot."""
+    nt = WORKLOADS[name][4]
+    k = nt if steps is None else min(steps, nt)
+    sec, _ = _run_once(name, k, order, backward=False)
+    return sec / k * 1e3
 
 
 def time_sample(name, steps=None, warm=True, order=None):
@@ -127,7 +138,9 @@ def main():
         raise SystemExit("oracle/_ref is not built (oracle/build_ref.sh needs the reference tree)")
     for name in args.workloads:
         steps = None if name == "acoustic2d_small" else args.steps
-        print(json.dumps(time_sample(name, steps)), flush=True)
+        out = time_sample(name, steps)
+        out["fwd_ms_per_step"] = time_forward(name, steps)
+        print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -22,7 +22,7 @@ _F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.POINTER(ctypes.c_int32)
 
 MEM_NONE, MEM_FULL, MEM_BOUNDARY, MEM_CKPT, MEM_BINOMIAL = 0, 1, 2, 3, 4
-FLAG_ILLUMINATION, FLAG_BF16_STRIPS = 1, 2
+FLAG_ILLUMINATION, FLAG_BF16_STRIPS, FLAG_GRAPH = 1, 2, 4
 
 
 class SwpProblem(ctypes.Structure):
@@ -75,7 +75,7 @@ EXPORTS = (
     "swp_nn_conv_fwd", "swp_nn_conv_wgrad_scratch", "swp_nn_conv_wgrad", "swp_nn_conv_flip", "swp_nn_relu_bwd",
     "swp_nn_pool2_fwd", "swp_nn_pool2_bwd_acc", "swp_nn_up2_fwd", "swp_nn_up2_bwd", "swp_nn_pack_input",
     "swp_nn_mse", "swp_nn_adam", "swp_nn_to_bf16", "swp_set_tuning", "swp_get_tuning",
-    "swp_census_enable", "swp_census",
+    "swp_census_enable", "swp_census", "swp_set_graphs", "swp_graph_stats", "swp_graph_clear",
 )
 TUNE_TZ2, TUNE_TZ2_BAND, TUNE_TZ2_RECON, TUNE_TZ3 = 0, 1, 2, 3
 
@@ -144,6 +144,12 @@ def _declare(L):
     L.swp_get_tuning.argtypes = [_i]
     L.swp_nn_conv_wgrad_scratch.restype = _i64
     L.swp_nn_conv_wgrad_scratch.argtypes = [_i, _i, _i, _i, _i, _i]
+    L.swp_set_graphs.restype = None
+    L.swp_set_graphs.argtypes = [_c]
+    L.swp_graph_stats.restype = _c
+    L.swp_graph_stats.argtypes = [ctypes.POINTER(_i64)] * 4
+    L.swp_graph_clear.restype = None
+    L.swp_graph_clear.argtypes = []
     L.swp_census_enable.restype = None
     L.swp_census_enable.argtypes = [_c]
     L.swp_census.restype = _i64
@@ -261,6 +267,23 @@ def get_tuning(key: int) -> int:
     if v < 0:
         raise ValueError(f"bad tuning key {key}")
     return int(v)
+
+
+def set_graphs(mode: int):
+    """Time-loop graphs: 1 on, 0 off, -1 the SWP_GRAPHS environment default."""
+    load().swp_set_graphs(int(mode))
+
+
+def graph_stats() -> dict:
+    """Captures, replays, eager fallbacks and cached graphs since graph_clear()."""
+    v = [ctypes.c_int64(0) for _ in range(4)]
+    load().swp_graph_stats(*[ctypes.byref(x) for x in v])
+    return dict(zip(("captures", "replays", "eager", "cached"), (int(x.value) for x in v)))
+
+
+def graph_clear():
+    """Free every cached graph and reset the counts."""
+    load().swp_graph_clear()
 
 
 def census_enable(on: bool = True):

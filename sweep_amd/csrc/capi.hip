@@ -25,6 +25,32 @@ namespace {
 thread_local hipStream_t t_stream = nullptr;
 std::atomic<int> g_tuning[SWP_TUNE_COUNT];
 
+// ---- time-loop graphs
+// A call's whole launch sequence (every step of the forward or backward
+// loop: a few kernels per step, thousands of steps) is captured once into a
+// HIP graph and replayed when the same call comes again: same entry,
+// device, problem (pointers and scalars, the host coefficient values),
+// io pointers, tuning and kernel-family settings.  Replays are bit-identical
+// to the eager launches (the same kernels with the same arguments in the same
+// stream order).  A problem whose buffers move between calls never hits: after
+// kGraphMissLimit captures of one shape without a replay it runs eagerly.
+std::atomic<int> g_graph_mode{-1};   // -1: SWP_GRAPHS (default on), 0 off, 1 on
+std::mutex g_graph_mu;
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    uint64_t used = 0;
+};
+struct ShapeStats {
+    int captures = 0, replays = 0;
+};
+std::map<std::string, GraphEntry> g_graphs;
+std::map<std::string, ShapeStats> g_graph_shapes;
+uint64_t g_graph_clock = 0;
+int64_t g_graph_counts[3] = {0, 0, 0};   // captures, replays, eager fallbacks
+constexpr size_t kGraphCacheMax = 16;
+constexpr int kGraphMissLimit = 3;
+thread_local std::map<int, hipStream_t> t_capture_streams;
+
 std::atomic<bool> g_census_on{false};
 std::mutex g_census_mu;
 std::map<std::string, std::pair<int64_t, int64_t>> g_census;   // site -> (launches, workgroups)
@@ -119,6 +145,164 @@ void census_add(const char *site, int64_t blocks)
     e.second += blocks;
 }
 
+namespace {
+
+// Every launch of one call, on the thread's stream.
+void run_entry(const Entry &e, const swp_problem &p, void *io)
+{
+    if (e.kind == 0) {
+        auto *f = static_cast<swp_forward_io *>(io);
+        if (e.equation == 0) a2d::forward(a2d::make_ctx(p), p, f->record);
+        if (e.equation == 1) a3d::forward(p, f->record);
+        if (e.equation == 2) e2d::forward(p, f->record);
+    } else {
+        auto *b = static_cast<swp_backward_io *>(io);
+        if (e.equation == 0) a2d::backward(a2d::make_ctx(p), p, *b);
+        if (e.equation == 1) a3d::backward(p, *b);
+        if (e.equation == 2) e2d::backward(p, *b);
+    }
+}
+
+bool graphs_on()
+{
+    int m = g_graph_mode.load();
+    if (m < 0) {
+        const char *e = std::getenv("SWP_GRAPHS");
+        m = (e && std::string(e) == "0") ? 0 : 1;
+    }
+    return m == 1;
+}
+
+template <class T>
+void put(std::string &k, const T &v)
+{
+    k.append(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+
+void put_env(std::string &k, const char *name)
+{
+    const char *v = std::getenv(name);
+    k += name;
+    k += '=';
+    k += v ? v : "";
+    k += ';';
+}
+
+// (shape key, full key) of a call: everything the launches depend on, field
+// by field (no struct padding).  The shape key leaves out the buffer
+// addresses.
+std::pair<std::string, std::string> graph_keys(int entry, int dev, const swp_problem &p, const void *io, int kind)
+{
+    std::string k;
+    for (int v : {entry, dev, p.ndim, p.nz, p.ny, p.nx, p.B, p.M, p.nt, p.abcn, p.model_batched, p.nsrc, p.nrec,
+                  p.memory, p.ckpt_interval, p.flags, p.nsrc_fields, p.nrec_fields})
+        put(k, v);
+    for (int v : p.pad) put(k, v);
+    for (int v : p.src_fields) put(k, v);
+    for (int v : p.rec_fields) put(k, v);
+    for (float v : {p.dt, p.dz, p.dy, p.dx}) put(k, v);
+    put(k, p.tape_bytes);
+    put(k, p.workspace_bytes);
+    for (int i = 0; i <= p.M && i < 16; ++i) {
+        put(k, p.lap_coes ? p.lap_coes[i] : 0.f);
+        put(k, p.grad_coes ? p.grad_coes[i] : 0.f);
+    }
+    for (int i = 0; i < SWP_TUNE_COUNT; ++i) put(k, g_tuning[i].load());
+    for (const char *v : {"SWP_KERNELS", "SWP_SHELL3", "SWP_TZ", "SWP_TZ_BAND", "SWP_TZ_RECON", "SWP_TZ3"})
+        put_env(k, v);
+    std::string full = k;
+    for (const void *v : {static_cast<const void *>(p.source), static_cast<const void *>(p.sources_loc),
+                          static_cast<const void *>(p.receivers_loc), static_cast<const void *>(p.tape),
+                          static_cast<const void *>(p.workspace)})
+        put(full, v);
+    for (const float *v : p.models) put(full, v);
+    for (const float *v : p.pml) put(full, v);
+    if (kind == 0) {
+        put(full, static_cast<const swp_forward_io *>(io)->record);
+    } else {
+        const auto *b = static_cast<const swp_backward_io *>(io);
+        put(full, b->adjoint_source);
+        put(full, b->grad_source);
+        for (const float *v : b->grad_models) put(full, v);
+        put(full, b->illum_source);
+        put(full, b->illum_receiver);
+    }
+    return {k, full};
+}
+
+hipStream_t capture_stream(int dev)
+{
+    auto it = t_capture_streams.find(dev);
+    if (it != t_capture_streams.end()) return it->second;
+    hipStream_t s = nullptr;
+    SWP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    t_capture_streams[dev] = s;
+    return s;
+}
+
+// Run the call through the graph cache: replay a captured graph of the same
+// call, or capture this one (on a side stream, nothing runs while capturing)
+// and launch it.  Returns false when the call should run eagerly instead.
+bool run_graphed(int entry, const Entry &e, const swp_problem &p, void *io)
+{
+    if (!(p.flags & SWP_FLAG_GRAPH) || !graphs_on() || timer().on || census_on()) return false;
+    int dev = 0;
+    SWP_HIP(hipGetDevice(&dev));
+    const auto keys = graph_keys(entry, dev, p, io, e.kind);
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        auto it = g_graphs.find(keys.second);
+        if (it != g_graphs.end()) {
+            it->second.used = ++g_graph_clock;
+            g_graph_shapes[keys.first].replays += 1;
+            g_graph_counts[1] += 1;
+            exec = it->second.exec;
+        } else {
+            const ShapeStats &st = g_graph_shapes[keys.first];
+            if (st.replays == 0 && st.captures >= kGraphMissLimit) {
+                g_graph_counts[2] += 1;
+                return false;
+            }
+        }
+    }
+    if (!exec) {
+        const hipStream_t user = stream();
+        const hipStream_t cap = capture_stream(dev);
+        hipGraph_t graph = nullptr;
+        t_stream = cap;
+        SWP_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        try {
+            run_entry(e, p, io);
+        } catch (...) {
+            hipStreamEndCapture(cap, &graph);
+            if (graph) hipGraphDestroy(graph);
+            t_stream = user;
+            throw;
+        }
+        t_stream = user;
+        SWP_HIP(hipStreamEndCapture(cap, &graph));
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        SWP_CHECK(ie == hipSuccess, "hipGraphInstantiate: ", hipGetErrorString(ie));
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        if (g_graphs.size() >= kGraphCacheMax) {
+            auto old = g_graphs.begin();
+            for (auto it = g_graphs.begin(); it != g_graphs.end(); ++it)
+                if (it->second.used < old->second.used) old = it;
+            hipGraphExecDestroy(old->second.exec);
+            g_graphs.erase(old);
+        }
+        g_graphs[keys.second] = GraphEntry{exec, ++g_graph_clock};
+        g_graph_shapes[keys.first].captures += 1;
+        g_graph_counts[0] += 1;
+    }
+    SWP_HIP(hipGraphLaunch(exec, stream()));
+    return true;
+}
+
+}  // namespace
+
 }  // namespace swp
 
 using namespace swp;
@@ -209,6 +393,7 @@ SWP_CAPI int swp_query(int entry, const swp_problem *in, int64_t *workspace_byte
     });
 }
 
+
 SWP_CAPI int swp_call(int entry, const swp_problem *in, void *io, char *err, int err_cap)
 {
     return guarded(err, err_cap, [&] {
@@ -225,20 +410,37 @@ SWP_CAPI int swp_call(int entry, const swp_problem *in, void *io, char *err, int
                   p.workspace_bytes);
         SWP_CHECK(p.memory == SWP_MEM_NONE || (p.tape && p.tape_bytes >= tp), e.name, ": tape needs ", tp,
                   " bytes, got ", p.tape_bytes);
-        if (e.kind == 0) {
-            auto *f = static_cast<swp_forward_io *>(io);
-            SWP_CHECK(p.nrec == 0 || f->record, e.name, ": record buffer missing");
-            if (e.equation == 0) a2d::forward(a2d::make_ctx(p), p, f->record);
-            if (e.equation == 1) a3d::forward(p, f->record);
-            if (e.equation == 2) e2d::forward(p, f->record);
-        } else {
-            auto *b = static_cast<swp_backward_io *>(io);
-            SWP_CHECK(p.nrec == 0 || b->adjoint_source, e.name, ": adjoint source missing");
-            if (e.equation == 0) a2d::backward(a2d::make_ctx(p), p, *b);
-            if (e.equation == 1) a3d::backward(p, *b);
-            if (e.equation == 2) e2d::backward(p, *b);
-        }
+        if (e.kind == 0)
+            SWP_CHECK(p.nrec == 0 || static_cast<swp_forward_io *>(io)->record, e.name, ": record buffer missing");
+        else
+            SWP_CHECK(p.nrec == 0 || static_cast<swp_backward_io *>(io)->adjoint_source, e.name,
+                      ": adjoint source missing");
+        if (!run_graphed(entry, e, p, io)) run_entry(e, p, io);
     });
+}
+
+SWP_CAPI void swp_set_graphs(int mode)
+{
+    g_graph_mode.store(mode < 0 ? -1 : (mode ? 1 : 0));
+}
+
+SWP_CAPI int swp_graph_stats(int64_t *captures, int64_t *replays, int64_t *eager, int64_t *cached)
+{
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    if (captures) *captures = g_graph_counts[0];
+    if (replays) *replays = g_graph_counts[1];
+    if (eager) *eager = g_graph_counts[2];
+    if (cached) *cached = int64_t(g_graphs.size());
+    return 0;
+}
+
+SWP_CAPI void swp_graph_clear(void)
+{
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    for (auto &kv : g_graphs) hipGraphExecDestroy(kv.second.exec);
+    g_graphs.clear();
+    g_graph_shapes.clear();
+    for (auto &c : g_graph_counts) c = 0;
 }
 
 SWP_CAPI void swp_set_timing(int stride)

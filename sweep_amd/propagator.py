@@ -58,7 +58,50 @@ class _Call:
     device: torch.device = None
     memory: int = 0
     requires_backward: bool = False
+    arena: object = None
     illumination: bool = False
+
+
+class _Arena:
+    """Device buffers of one call shape kept from call to call (workspace,
+    tape, raw record, and staging copies of the inputs and outputs), so the
+    core sees the same addresses every time and replays the HIP graph it
+    captured of the call's time loop (SWP_FLAG_GRAPH).  A forward that keeps
+    a tape for its backward holds the arena until that backward ran or its
+    autograd graph was dropped."""
+
+    def __init__(self, dev):
+        self.dev = dev
+        self.bufs = {}
+        self.busy = False
+
+    def get(self, name, shape, dtype=torch.float32):
+        t = self.bufs.get(name)
+        if t is None or tuple(t.shape) != tuple(shape) or t.dtype != dtype:
+            t = torch.empty(tuple(shape), dtype=dtype, device=self.dev)
+            self.bufs[name] = t
+        return t
+
+    def get_bytes(self, name, n):
+        """A uint8 buffer of at least n bytes (grown, never shrunk)."""
+        t = self.bufs.get(name)
+        if t is None or t.numel() < n:
+            self.bufs.pop(name, None)
+            t = torch.empty(max(int(n), 1), dtype=torch.uint8, device=self.dev)
+            self.bufs[name] = t
+        return t
+
+    def stage(self, name, src):
+        t = self.get(name, src.shape, src.dtype)
+        t.copy_(src)
+        return t
+
+    def release(self):
+        self.busy = False
+
+
+def _release_arena(arena):
+    arena.release()
 
 
 class _Wrapper(torch.autograd.Function):
@@ -67,23 +110,44 @@ class _Wrapper(torch.autograd.Function):
     def forward(ctx, call: _Call, wavelet, *models):
         P = call.problem
         dev = call.device
-        P.source = _ptr(wavelet)
-        for k, m in enumerate(models):
-            P.models[k] = _ptr(m)
+        A = call.arena
         P.memory = call.memory if call.requires_backward else _core.MEM_NONE
-        ws, tp = _core.query(call.names[0], P)
-        workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
-        tape = torch.empty(tp, dtype=torch.uint8, device=dev) if P.memory != _core.MEM_NONE else None
-        P.workspace, P.workspace_bytes = workspace.data_ptr(), ws
-        P.tape, P.tape_bytes = (tape.data_ptr(), tp) if tape is not None else (None, 0)
         B, nt = P.B, P.nt
-        record = torch.zeros((call.nfield, B, call.nrec, nt) if call.nfield > 1 else (B, call.nrec, nt),
-                             dtype=torch.float32, device=dev)
+        rshape = (call.nfield, B, call.nrec, nt) if call.nfield > 1 else (B, call.nrec, nt)
+        if A is not None:
+            # the arena's fixed buffers: stage the inputs, replay the graph
+            P.flags |= _core.FLAG_GRAPH
+            P.source = _ptr(A.stage("wavelet", wavelet))
+            for k, m in enumerate(models):
+                P.models[k] = _ptr(A.stage(f"model{k}", m))
+            P.sources_loc = _iptr(A.stage("sources_loc", call.keep[0]))
+            P.receivers_loc = _iptr(A.stage("receivers_loc", call.keep[1]))
+            ws, tp = _core.query(call.names[0], P)
+            workspace = A.get_bytes("workspace", ws)
+            tape = A.get_bytes("tape", tp) if P.memory != _core.MEM_NONE else None
+            record = A.get("record", rshape)
+            record.zero_()
+            if tape is not None:
+                A.busy = True
+        else:
+            P.source = _ptr(wavelet)
+            for k, m in enumerate(models):
+                P.models[k] = _ptr(m)
+            ws, tp = _core.query(call.names[0], P)
+            workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
+            tape = torch.empty(tp, dtype=torch.uint8, device=dev) if P.memory != _core.MEM_NONE else None
+            record = torch.zeros(rshape, dtype=torch.float32, device=dev)
+        P.workspace, P.workspace_bytes = workspace.data_ptr(), workspace.numel()
+        P.tape, P.tape_bytes = (tape.data_ptr(), tape.numel()) if tape is not None else (None, 0)
         io = _core.SwpForwardIO(record=_ptr(record))
         call.eq._C()[0](P, io, dev)
         ctx.call = call
         ctx.tape = tape
         ctx.used = False
+        if A is not None and tape is not None:
+            # the tape's arena is free again once this graph is gone
+            import weakref
+            weakref.finalize(ctx, _release_arena, A)
         ctx.save_for_backward(wavelet, *models)
         return records.to_canonical(record)
 
@@ -106,14 +170,25 @@ class _Wrapper(torch.autograd.Function):
         name = call.names[entry]
         if call.illumination:
             P.flags |= _core.FLAG_ILLUMINATION   # before the query: the workspace depends on it
+        A = call.arena
         ws, _ = _core.query(name, P)
-        workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
-        P.workspace, P.workspace_bytes = workspace.data_ptr(), ws
         grid = (P.B,) + ((P.nz, P.ny, P.nx) if P.ndim == 3 else (P.nz, P.nx))
         want_w = ctx.needs_input_grad[1]
-        g_w = torch.zeros_like(wavelet) if want_w else None
-        g_m = [torch.zeros(grid, dtype=torch.float32, device=dev) if ctx.needs_input_grad[2 + k] else None
-               for k in range(len(models))]
+        want_m = [ctx.needs_input_grad[2 + k] for k in range(len(models))]
+
+        def out_buf(name, shape):
+            if A is None:
+                return torch.zeros(shape, dtype=torch.float32, device=dev)
+            t = A.get(name, shape)
+            t.zero_()
+            return t
+
+        workspace = A.get_bytes("workspace", ws) if A is not None else torch.empty(ws, dtype=torch.uint8, device=dev)
+        P.workspace, P.workspace_bytes = workspace.data_ptr(), workspace.numel()
+        g_w = out_buf("grad_wavelet", wavelet.shape) if want_w else None
+        g_m = [out_buf(f"grad_model{k}", grid) if want_m[k] else None for k in range(len(models))]
+        if A is not None:
+            adj = A.stage("adjoint_source", adj)
         io = _core.SwpBackwardIO()
         io.adjoint_source = _ptr(adj)
         io.grad_source = _ptr(g_w)
@@ -121,12 +196,18 @@ class _Wrapper(torch.autograd.Function):
             io.grad_models[k] = _ptr(g)
         S = R = None
         if call.illumination:
-            S = torch.zeros(grid, dtype=torch.float32, device=dev)
-            R = torch.zeros(grid, dtype=torch.float32, device=dev)
+            S = out_buf("illum_source", grid)
+            R = out_buf("illum_receiver", grid)
             io.illum_source, io.illum_receiver = _ptr(S), _ptr(R)
         P.tape, P.tape_bytes = ctx.tape.data_ptr(), ctx.tape.numel()
         call.eq._C()[entry](P, io, dev)
         ctx.tape = None
+        if A is not None:
+            # the caller gets tensors of its own; the arena's are reused
+            g_w = g_w.clone() if g_w is not None else None
+            g_m = [g.clone() if g is not None else None for g in g_m]
+            S, R = (S.clone(), R.clone()) if S is not None else (None, None)
+            A.release()
         if call.illumination:
             call.illum_out = (S, R)
         out_m = []
@@ -146,7 +227,7 @@ class PropTorch(torch.nn.Module):
     def __init__(self, equation, shape, dh=10.0, dt=0.002, device=None, dev=None, impl=None,
                  memory=None, abcn=50, free_surface=False, nt=-1, B=1, source_type=None,
                  receiver_type=None, compute_illumination=False, use_ckpt=None, topography=None,
-                 backend=None, cuda_options=None, eager_options=None, **kwargs):
+                 backend=None, cuda_options=None, eager_options=None, graphs=True, **kwargs):
         super().__init__()
         if not isinstance(equation, WaveEquation):
             raise TypeError("equation must be an Acoustic / Acoustic3D / Elastic instance")
@@ -213,6 +294,11 @@ class PropTorch(torch.nn.Module):
         self.shape = tuple(self._shape_phys[a] + pad[2 * a] + pad[2 * a + 1] for a in range(self.ndim))
         self.shape_cuda = tuple(s + 2 * M for s in self.shape)
         self._abc_cache = {}
+        # graphs=True: calls run on per-shape arenas of fixed buffers and the
+        # core replays each call's captured time-loop graph (an extension:
+        # the reference launches step by step); False allocates per call
+        self.graphs = bool(graphs)
+        self._arenas = {}
         self.funcs = equation._C()
         self.__dict__["_built"] = True
 
@@ -409,9 +495,27 @@ class PropTorch(torch.nn.Module):
                      nrec=nrec, nfield=len(self.receiver_type) if isinstance(eq, Elastic) else 1,
                      device=dev, memory=int(P.memory), requires_backward=bool(requires),
                      illumination=self.compute_illumination)
+        if self.graphs:
+            call.arena = self._arena_for((B, nsrc, nrec, nt, bool(shared), int(P.memory), int(P.flags),
+                                          int(P.ckpt_interval), call.requires_backward, len(models)))
         self._last_call = call
         out = _Wrapper.apply(call, w, *padded)
         return out
+
+    _ARENAS_PER_SHAPE = 2
+
+    def _arena_for(self, key):
+        """A free arena for calls of this shape, or None (per-call buffers)
+        when every arena of the shape holds a tape that is still waiting
+        for its backward."""
+        pool = self._arenas.setdefault(key, [])
+        for a in pool:
+            if not a.busy:
+                return a
+        if len(pool) < self._ARENAS_PER_SHAPE:
+            pool.append(_Arena(self.dev))
+            return pool[-1]
+        return None
 
     def collect_illumination(self):
         """Source (sum u_tt^2) and receiver (sum lambda^2) illumination of the

@@ -259,7 +259,8 @@ def test_acoustic3d_full_grid_modes_agree():
 def test_acoustic2d_full_grid_modes_agree(zmarch):
     """One shot on the 384 x 1152 workload grid (runtime 492 x 1260, order 8)
     on the z-march, nt 1500: boundary saving and binomial checkpointing
-    against full storage, and the adjoint identity."""
+    against full storage (records bit for bit, gradients to 1e-5 and 1e-4),
+    and the adjoint identity."""
     dev = _dev()
     from sweep_amd import synthetic
     from sweep_amd.equations import Acoustic
@@ -271,12 +272,18 @@ def test_acoustic2d_full_grid_modes_agree(zmarch):
     eq = Acoustic(spatial_order=8, device=dev)
     out = {mode: _run(eq, shape, dh, dt, 50, mode, vp, wav, src, rec, res, dev) for mode in ("full", "bs", "binomial6")}
     assert out["full"]["census"].get("a2d.adj_int", (0, 0))[1] > 0
-    for k in ("rec", "grad_vp", "grad_wavelet"):
-        assert np.array_equal(out["full"][k], out["binomial6"][k]), k
+    # the binomial replay reproduces the forward bit for bit; its adjoint
+    # steps run the kernel variant without fused imaging (the image is a
+    # separate pass), which rounds the lambda update differently from the
+    # fused variant full storage runs, so the gradients agree to rounding
+    assert np.array_equal(out["full"]["rec"], out["binomial6"]["rec"])
     assert np.array_equal(out["full"]["rec"], out["bs"]["rec"])
+    eb = {k: rel_l2(out["binomial6"][k], out["full"][k]) for k in ("grad_vp", "grad_wavelet")}
     e = rel_l2(out["bs"]["grad_vp"], out["full"]["grad_vp"])
     d, lhs, rhs = _dot_identity(out["full"], wav, res)
-    print(f"[c2grid] bs vs full grad_vp {e:.2e}; <Fw,r> {lhs:.9e} <w,F^T r> {rhs:.9e} rel {d:.2e}", flush=True)
+    print(f"[c2grid] binomial vs full {eb}; bs vs full grad_vp {e:.2e}; <Fw,r> {lhs:.9e} <w,F^T r> {rhs:.9e} "
+          f"rel {d:.2e}", flush=True)
+    assert eb["grad_vp"] <= 1e-5 and eb["grad_wavelet"] <= 1e-5
     assert e <= 1e-4
     assert d <= 2e-5
 
@@ -286,10 +293,15 @@ def test_acoustic2d_workload_gradient_fd():
     of 32 surface shots with uneven receiver spreads (150-500) on the
     384 x 1152 grid at nt 3000, against central differences at 20 random
     cells.  J(m) = <record(m), r> for a fixed random r with the padded
-    receivers masked out, accumulated in float64; each perturbation is a
-    Gaussian bump (sigma 2 cells, 1% of the local model), compared with
-    <grad, bump>.  Bars as in test_gpu_fd.py: correlation >= 0.99995, median
-    relative error <= 1%, every error <= 2% of the rms derivative."""
+    receivers masked out, accumulated in float64.  r is random noise
+    filtered to the source wavelet's band: the float32 rounding differences
+    between the two perturbed runs are broadband, the record's response to
+    the perturbation is not, so an in-band r keeps J's difference above the
+    rounding noise (with white r the noise alone reached ~1% at a 1% bump and
+    grew as the bump shrank).  Each perturbation is a Gaussian bump (sigma 2
+    cells, 1% of the local model), compared with <grad, bump>.  Bars as in
+    test_gpu_fd.py: correlation >= 0.99995, median relative error <= 1%,
+    every error <= 2% of the rms derivative."""
     dev = _dev()
     from sweep_amd import synthetic
     from sweep_amd.equations import Acoustic
@@ -303,8 +315,11 @@ def test_acoustic2d_workload_gradient_fd():
     solver = PropTorch(Acoustic(spatial_order=8, device=dev), shape=shape, dh=dh, dt=dt, device=dev, abcn=50,
                        memory=BoundarySaving())
     rng = np.random.default_rng(20)
-    r = rng.standard_normal((32, nt, rec.shape[1], 1)).astype(np.float32) * mask[:, None, :, None]
-    r = torch.tensor(r, device=dev)
+    white = rng.standard_normal((32, rec.shape[1], nt))
+    band = np.abs(np.fft.rfft(wav.astype(np.float64)))
+    r = np.fft.irfft(np.fft.rfft(white, axis=-1) * band, n=nt, axis=-1)
+    r = (r / r.std()).transpose(0, 2, 1)[..., None] * mask[:, None, :, None]
+    r = torch.tensor(r.astype(np.float32), device=dev)
     m = torch.tensor(vp, device=dev, requires_grad=True)
     (solver(wav, src, rec, models=[m]) * r).sum().backward()
     g = m.grad.cpu().numpy().astype(np.float64)
