@@ -134,11 +134,11 @@ __device__ __forceinline__ bool zcoords(const ZSet &s, int &b, int &k, int &xo, 
 // ---------------------------------------------------------------- plane I/O
 // The halo duties of a lane: y-halo rows j = w + 4t (t < YH) of the 2M rows
 // above and below the tile, x-halo columns for lanes < 2M on each of its rows.
-template <int M, int TY>
+template <int M, int TY, int NW>
 struct PlaneMap {
     static constexpr int LW = 64 + 2 * M;
     static constexpr int LH = TY + 2 * M;
-    static constexpr int YH = (2 * M + 3) / 4;
+    static constexpr int YH = (2 * M + NW - 1) / NW;
     int yh_row[YH];      // global row of each y-halo duty (clamped), or -1
     int yh_slot[YH];     // LDS row of each duty
     int xh_col, xh_slot; // x-halo column (clamped) and LDS column, xh_slot < 0: none
@@ -146,7 +146,7 @@ struct PlaneMap {
     {
 #pragma unroll
         for (int t = 0; t < YH; ++t) {
-            const int j = w + 4 * t;
+            const int j = w + NW * t;
             if (j < 2 * M) {
                 const int gy = j < M ? yo - M + j : yo + TY + (j - M);
                 yh_row[t] = min(max(gy, 0), ny - 1);
@@ -170,7 +170,7 @@ struct PlaneMap {
     }
 };
 
-template <int M, int RY>
+template <int M, int RY, int NW>
 struct Lanes {
     int x, xc;
     bool xa;
@@ -183,7 +183,7 @@ struct Lanes {
         xc = min(x, nx - 1);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            y[r] = yo + w + 4 * r;
+            y[r] = yo + w + NW * r;
             yc[r] = min(y[r], ny - 1);
             act[r] = xa && y[r] < R.y1;
         }
@@ -196,7 +196,7 @@ struct Lanes {
 // left of the tile, then M on the right), so each lane holds at most one
 // x-halo value per plane.  The y-halo rows are spread as in
 // PlaneMap (rows j = w + 4t of the 2M rows above and below the tile).
-template <int M, int RY>
+template <int M, int RY, int NW>
 struct XHalo {
     static_assert(2 * M * RY <= 64, "x-halo duties exceed one wave");
     uint32_t off;   // in-plane byte offset (clamped into the grid)
@@ -206,7 +206,7 @@ struct XHalo {
     {
         if (lane < 2 * M * RY) {
             const int r = lane / (2 * M), c = lane % (2 * M);
-            const int row = w + 4 * r;
+            const int row = w + NW * r;
             const int gx = c < M ? xo - M + c : xo + 64 + (c - M);
             const int gy = min(yo + row, ny - 1);
             col = min(max(gx, 0), nx - 1);
@@ -237,13 +237,13 @@ struct XHalo {
 // (when the neighbouring tiles fetch that plane as their own head, so the
 // shared lines are still in L2) and waits in an (M + 1)-deep queue until the
 // plane is staged in LDS; halo slot p mod (M + 1) holds plane zb + p.
-template <int M, int RY, bool BX, bool BY, bool BZ>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) fwd3_kernel(Geo g, Coef c, Prof pr, StepFields f, float *__restrict__ u_this,
+template <int M, int RY, bool BX, bool BY, bool BZ, int NW>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) fwd3_kernel(Geo g, Coef c, Prof pr, StepFields f, float *__restrict__ u_this,
                                                    float *__restrict__ strip, ZSet zs)
 {
     constexpr bool SHELL = BX || BY || BZ;
-    constexpr int TY = 4 * RY;
-    using PM = PlaneMap<M, TY>;
+    constexpr int TY = NW * RY;
+    using PM = PlaneMap<M, TY, NW>;
     constexpr int LW = PM::LW, PS = PM::LH * PM::LW, YH = PM::YH;
     constexpr int NPL = 1 + (BX ? 1 : 0) + (BY ? 1 : 0);
     constexpr int SX = 1, SY = BX ? 2 : 1;   // plane slots of psi_x, psi_y
@@ -255,9 +255,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nx = g.nx, ny = g.ny;
     const int64_t sxy = g.sxy;
-    const Lanes<M, RY> L(Rc, lane, w, xo, yo, nx, ny);
+    const Lanes<M, RY, NW> L(Rc, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
-    const XHalo<M, RY> xh(lane, w, xo, yo, nx, g.px, ny);
+    const XHalo<M, RY, NW> xh(lane, w, xo, yo, nx, g.px, ny);
     const int64_t off = int64_t(b) * g.n;
     const float dt = g.dt;
     // fields from plane zb - M on; plane z sits at byte offset pl(z)
@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
     auto store_plane = [&](float *P, int cs, int hs) {
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            const int row = (w + 4 * r + M) * LW;
+            const int row = (w + NW * r + M) * LW;
             P[row + lane + M] = q[r][cs];
             if constexpr (BX) P[SX * PS + row + lane + M] = npx[r];
             if constexpr (BY) P[SY * PS + row + lane + M] = npy[r];
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             if (!L.act[r]) continue;
-            const int row = (w + 4 * r + M) * LW + lane + M;
+            const int row = (w + NW * r + M) * LW + lane + M;
             const float *Rw = P + row;
             const float u0 = q[r][C];
             auto qz = [&](int t) { return q[r][(C + t + R) % R]; };
@@ -506,12 +506,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) f
 // -- the expression tree of the per-cell adjoint (adjoint_split.cu:56-318)
 // with each neighbour's T formed once.  img != nullptr fuses the lagged
 // full-mode imaging grad += 2 dt^2 img lambda / v.
-template <int M, int RY, bool BX, bool BY, bool BZ>
-__global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFields f, const float *__restrict__ img, float *__restrict__ grad, ZSet zs)
+template <int M, int RY, bool BX, bool BY, bool BZ, int NW>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 1))) adj3_kernel(Geo g, Coef c, Prof pr, AdjFields f, const float *__restrict__ img, float *__restrict__ grad, ZSet zs)
 {
     constexpr bool SHELL = BX || BY || BZ;
-    constexpr int TY = 4 * RY;
-    using PM = PlaneMap<M, TY>;
+    constexpr int TY = NW * RY;
+    using PM = PlaneMap<M, TY, NW>;
     constexpr int LW = PM::LW, PS = PM::LH * PM::LW, YH = PM::YH;
     constexpr int NPL = 1 + (BX ? 3 : 0) + (BY ? 3 : 0);
     constexpr int SX = 1, SY = BX ? 4 : 1;   // first plane slot of (U, V, W) for x, y
@@ -523,9 +523,9 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nx = g.nx, ny = g.ny;
     const int64_t sxy = g.sxy;
-    const Lanes<M, RY> L(Rc, lane, w, xo, yo, nx, ny);
+    const Lanes<M, RY, NW> L(Rc, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
-    const XHalo<M, RY> xh(lane, w, xo, yo, nx, g.px, ny);
+    const XHalo<M, RY, NW> xh(lane, w, xo, yo, nx, g.px, ny);
     const int64_t off = int64_t(b) * g.n;
     const float dt2 = g.dt2;
     // fields from plane zb - M on; plane z sits at byte offset pl(z)
@@ -676,7 +676,7 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
     auto store_plane = [&](float *P, int cs, int hs) {
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            const int row = (w + 4 * r + M) * LW;
+            const int row = (w + NW * r + M) * LW;
             const float gc = qg[r][cs];
             P[row + lane + M] = gc;
             if constexpr (BX) {
@@ -764,7 +764,7 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             if (!L.act[r]) continue;
-            const int row = (w + 4 * r + M) * LW + lane + M;
+            const int row = (w + NW * r + M) * LW + lane + M;
             const float *Gw = P + row;
             const float gun = ql[r][C];
             const float gw = qg[r][C];
@@ -869,14 +869,14 @@ __global__ void __launch_bounds__(256) adj3_kernel(Geo g, Coef c, Prof pr, AdjFi
 // ---------------------------------------------------------------- BS recon
 // Reverse leapfrog without PML over one box, fused u_tt imaging on the cells
 // of [lo + W, hi - W) per axis (the cells the strip restore leaves alone).
-template <int M, int RY>
-__global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const float *__restrict__ vpb,
+template <int M, int RY, int NW>
+__global__ void __launch_bounds__(64 * NW) recon_zm_kernel(Geo g, Coef c, const float *__restrict__ vpb,
                                                        const float *__restrict__ u_prev, const float *__restrict__ u_now,
                                                        float *__restrict__ u_next, const float *__restrict__ adj,
                                                        float *__restrict__ grad, ZSet zs)
 {
-    constexpr int TY = 4 * RY;
-    using PM = PlaneMap<M, TY>;
+    constexpr int TY = NW * RY;
+    using PM = PlaneMap<M, TY, NW>;
     constexpr int LW = PM::LW;
     __shared__ float lds[2][PM::LH * PM::LW];
     int b, k, xo, yo, zb, ze;
@@ -885,7 +885,7 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nx = g.nx, ny = g.ny;
     const int64_t sxy = g.sxy;
-    const Lanes<M, RY> L(R, lane, w, xo, yo, nx, ny);
+    const Lanes<M, RY, NW> L(R, lane, w, xo, yo, nx, ny);
     const PM pm(lane, w, xo, yo, nx, ny);
     const int64_t off = int64_t(b) * g.n;
     const float dt = g.dt;
@@ -931,8 +931,8 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
     auto store_plane = [&](float *P, int cs) {
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            P[(w + 4 * r + M) * LW + lane + M] = q[r][cs];
-            if (pm.xh_slot >= 0) P[(w + 4 * r + M) * LW + pm.xh_slot] = nxh[r];
+            P[(w + NW * r + M) * LW + lane + M] = q[r][cs];
+            if (pm.xh_slot >= 0) P[(w + NW * r + M) * LW + pm.xh_slot] = nxh[r];
         }
 #pragma unroll
         for (int t = 0; t < PM::YH; ++t)
@@ -976,7 +976,7 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
             if (!L.act[r]) continue;
-            const float *Rw = P + (w + 4 * r + M) * LW + lane + M;
+            const float *Rw = P + (w + NW * r + M) * LW + lane + M;
             const float u0 = q[r][C];
             const float w_sum = lap_taps<M>(c, [&](int t) { return Rw[t]; }, g.inv_dx2) +
                                 lap_taps<M>(c, [&](int t) { return Rw[t * LW]; }, g.inv_dy2) +
@@ -999,10 +999,16 @@ __global__ void __launch_bounds__(256) recon_zm_kernel(Geo g, Coef c, const floa
 }
 
 // ---------------------------------------------------------------- host plans
-// Rows per lane of the interior kernels (a block's tile is 64 x 4*RY cells).
+// Rows per lane and waves per workgroup of the interior kernels (a block's
+// tile is 64 x NW*RY cells; each plane re-reads 2M halo rows per tile, so a
+// taller tile of more waves cuts the y-halo share without more registers per
+// lane).  The margin kernels keep 4 waves.
 constexpr int kRyFwd = 4;
 constexpr int kRyAdj = 4;
 constexpr int kRyRecon = 4;
+constexpr int kNwFwd = 8;
+constexpr int kNwAdj = 4;
+constexpr int kNwRecon = 8;
 
 ZRect make_rect(int x0, int x1, int y0, int y1, int z0, int z1, int tz, int TY, int px)
 {
@@ -1088,6 +1094,25 @@ void by_order(int M, K &&k)
     }
 }
 
+// The region set with its segment height: tz, halved (down to 16 planes)
+// while the launch would have fewer than `min_waves` waves (default 4096, 16
+// per CU; SWP_REGION_WAVES overrides, 1 keeps tz).  A block walks its
+// segment one plane (one load latency) at a time, so a region of few blocks
+// -- the edges and corners of the CPML margins -- runs for about tz plane
+// latencies however few cells it holds; shorter segments give it more
+// blocks that march fewer planes, at the cost of more 2M + 1 plane segment
+// prologues (which the larger regions avoid).
+ZSet region_set_sized(const Geo &g, bool ex, bool ey, bool ez, int tz, int TY, int NW)
+{
+    const int64_t min_waves = tuning(-1, "SWP_REGION_WAVES", 4096);
+    ZSet s = region_set(g, ex, ey, ez, tz, TY);
+    while (tz > 16 && int64_t(s.per_shot) * g.B * NW < min_waves) {
+        tz /= 2;
+        s = region_set(g, ex, ey, ez, tz, TY);
+    }
+    return s;
+}
+
 dim3 grid_of(const ZSet &s, int B)
 {
     const int64_t n = int64_t(s.per_shot) * B;
@@ -1106,13 +1131,14 @@ void fwd_region(const Ctx &x, const StepFields &f, float *u_this, float *strip)
 {
     constexpr bool SHELL = BX || BY || BZ;
     constexpr int RY = SHELL ? ry_fwd<BX, BY, BZ>() : kRyFwd;
-    const ZSet s = region_set(x.g, BX, BY, BZ, x.tz, 4 * RY);
+    constexpr int NW = SHELL ? 4 : kNwFwd;
+    const ZSet s = region_set_sized(x.g, BX, BY, BZ, x.tz, NW * RY, NW);
     if (!s.per_shot) return;
     if (census_on()) {
         const char site[] = {'a', '3', 'd', '.', 'f', 'w', 'd', '3', '.', char('0' + BX), char('0' + BY), char('0' + BZ), 0};
         census_add(site, int64_t(s.per_shot) * x.g.B);
     }
-    hipLaunchKernelGGL((fwd3_kernel<M, RY, BX, BY, BZ>), grid_of(s, x.g.B), dim3(256), 0, stream(), x.g, x.c, x.pr,
+    hipLaunchKernelGGL((fwd3_kernel<M, RY, BX, BY, BZ, NW>), grid_of(s, x.g.B), dim3(64 * NW), 0, stream(), x.g, x.c, x.pr,
                        f, u_this, strip, s);
 }
 
@@ -1121,13 +1147,14 @@ void adj_region(const Ctx &x, const AdjFields &f, const float *img, float *grad)
 {
     constexpr bool SHELL = BX || BY || BZ;
     constexpr int RY = SHELL ? ry_adj<BX, BY, BZ>() : kRyAdj;
-    const ZSet s = region_set(x.g, BX, BY, BZ, x.tz, 4 * RY);
+    constexpr int NW = SHELL ? 4 : kNwAdj;
+    const ZSet s = region_set_sized(x.g, BX, BY, BZ, x.tz, NW * RY, NW);
     if (!s.per_shot) return;
     if (census_on()) {
         const char site[] = {'a', '3', 'd', '.', 'a', 'd', 'j', '3', '.', char('0' + BX), char('0' + BY), char('0' + BZ), 0};
         census_add(site, int64_t(s.per_shot) * x.g.B);
     }
-    hipLaunchKernelGGL((adj3_kernel<M, RY, BX, BY, BZ>), grid_of(s, x.g.B), dim3(256), 0, stream(), x.g, x.c, x.pr,
+    hipLaunchKernelGGL((adj3_kernel<M, RY, BX, BY, BZ, NW>), grid_of(s, x.g.B), dim3(64 * NW), 0, stream(), x.g, x.c, x.pr,
                        f, img, grad, s);
 }
 
@@ -1199,12 +1226,12 @@ void launch_recon_zm(const Ctx &x, const float *vp, const float *u_prev, const f
 {
     const Geo &g = x.g;
     ZSet s{};
-    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, 4 * kRyRecon, x.g.px);
+    add_rect(s, x0, x1, y0, y1, z0, z1, x.tz, kNwRecon * kRyRecon, x.g.px);
     if (!s.per_shot) return;
     census_add("a3d.recon_zm", int64_t(s.per_shot) * g.B);
     by_order(g.M, [&](auto ms) {
         constexpr int M = decltype(ms)::value;
-        hipLaunchKernelGGL((recon_zm_kernel<M, kRyRecon>), grid_of(s, g.B), dim3(256), 0, stream(), g, x.c, vp, u_prev,
+        hipLaunchKernelGGL((recon_zm_kernel<M, kRyRecon, kNwRecon>), grid_of(s, g.B), dim3(64 * kNwRecon), 0, stream(), g, x.c, vp, u_prev,
                            u_now, u_next, adj, grad, s);
     });
     SWP_LAUNCHED();

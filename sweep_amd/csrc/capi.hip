@@ -208,7 +208,8 @@ std::pair<std::string, std::string> graph_keys(int entry, int dev, const swp_pro
         put(k, p.grad_coes ? p.grad_coes[i] : 0.f);
     }
     for (int i = 0; i < SWP_TUNE_COUNT; ++i) put(k, g_tuning[i].load());
-    for (const char *v : {"SWP_KERNELS", "SWP_SHELL3", "SWP_TZ", "SWP_TZ_BAND", "SWP_TZ_RECON", "SWP_TZ3"})
+    for (const char *v : {"SWP_KERNELS", "SWP_SHELL3", "SWP_TZ", "SWP_TZ_BAND", "SWP_TZ_RECON", "SWP_TZ3",
+                          "SWP_REGION_WAVES"})
         put_env(k, v);
     std::string full = k;
     for (const void *v : {static_cast<const void *>(p.source), static_cast<const void *>(p.sources_loc),
