@@ -156,8 +156,9 @@ SWP_CAPI void *swp_get_stream(void);
  * settings replays it instead of launching step by step.  Results are
  * bit-identical either way.  A shape whose buffers move between calls falls
  * back to eager launches after three captures without a replay.  Calls
- * without the flag, or with kernel timing or the launch census on, run
- * eagerly.
+ * without the flag, with kernel timing or the launch census on, or running
+ * more than 4096 time steps (re-runs of a checkpointed backward included)
+ * run eagerly.
  * swp_set_graphs: 1 on, 0 off, -1 the SWP_GRAPHS environment variable
  * (default on, "0" turns it off).  swp_graph_stats reports captures, replays,
  * eager fallbacks and cached graphs since the last swp_graph_clear, which

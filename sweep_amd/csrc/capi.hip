@@ -244,9 +244,24 @@ hipStream_t capture_stream(int dev)
 // Run the call through the graph cache: replay a captured graph of the same
 // call, or capture this one (on a side stream, nothing runs while capturing)
 // and launch it.  Returns false when the call should run eagerly instead.
+// Time steps a call runs (re-runs included): a captured graph holds a few
+// kernel nodes per step in one dependency chain, and a chain of ~10^5 nodes
+// (a binomial reversal with one snapshot slot re-runs O(nt^2) steps) is more
+// than graph instantiation handles, so such calls run eagerly.
+constexpr int64_t kGraphMaxSteps = 4096;
+
+int64_t call_steps(const Entry &e, const swp_problem &p)
+{
+    if (e.kind == 0) return p.nt;
+    if (p.memory == SWP_MEM_CKPT) return 2 * int64_t(p.nt);
+    if (p.memory == SWP_MEM_BINOMIAL) return int64_t(revolve_schedule(p.nt, std::max(1, p.ckpt_interval)).size());
+    return p.nt;
+}
+
 bool run_graphed(int entry, const Entry &e, const swp_problem &p, void *io)
 {
     if (!(p.flags & SWP_FLAG_GRAPH) || !graphs_on() || timer().on || census_on()) return false;
+    if (call_steps(e, p) > kGraphMaxSteps) return false;
     int dev = 0;
     SWP_HIP(hipGetDevice(&dev));
     const auto keys = graph_keys(entry, dev, p, io, e.kind);

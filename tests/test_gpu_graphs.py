@@ -122,3 +122,29 @@ def test_graphs_off_runs_eagerly():
     _core.graph_clear()
     _run(solver, models, wav, src, rec, res, grad_w, dev)
     assert _core.graph_stats()["captures"] == 0
+
+
+def test_long_reversals_run_eagerly():
+    """A binomial reversal with one snapshot slot re-runs O(nt^2) steps; the
+    core launches it step by step instead of capturing a graph of ~10^5
+    nodes (only the forward, nt steps, is captured)."""
+    dev = _dev()
+    from sweep_amd import _core, synthetic
+    from sweep_amd.equations import Acoustic
+    from sweep_amd.options import Ckpt
+    from sweep_amd.propagator import PropTorch
+    shape, nt, dt = (40, 48), 400, 1e-3
+    solver = PropTorch(Acoustic(spatial_order=8, device=dev), shape=shape, dh=10.0, dt=dt, device=dev, abcn=10,
+                       memory=Ckpt(mode="recursive", count=1))
+    vp = torch.full(shape, 2000.0, device=dev, requires_grad=True)
+    wav = synthetic.ricker(np.arange(nt) * dt - 0.08, 15.0)
+    src = np.array([[20, 3]])
+    rec = np.array([[[4 + 4 * k, 2] for k in range(8)]])
+    _core.set_graphs(1)
+    _core.graph_clear()
+    (solver(wav, src, rec, models=[vp]) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    st = _core.graph_stats()
+    print(f"[graphs] binomial, one slot: {st}", flush=True)
+    assert st["captures"] == 1, st
+    assert torch.isfinite(vp.grad).all() and vp.grad.abs().max() > 0
