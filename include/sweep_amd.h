@@ -248,6 +248,18 @@ SWP_CAPI int swp_l2_misfit(const float *pred, const float *obs, const float *mas
                            int32_t nrec, int32_t nt, float *residual, double *loss, double *partials,
                            char *err, int err_cap);
 
+/* swp_l2_misfit without the partials argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * partial sums go to a static device array of SWP_MISFIT_PARTIALS doubles
+ * (8 KiB) that the library holds on each GPU it runs on until the process
+ * ends; every block writes its own slot before the final sum reads it, so
+ * nothing needs clearing between calls.  Calls that overlap on one GPU (two
+ * streams, two threads) share that array and race: serialise them.  Same
+ * result as swp_l2_misfit bit for bit; allocates nothing. */
+SWP_CAPI int swp_l2_misfit_noalloc(const float *pred, const float *obs, const float *mask, int32_t nfield, int32_t B,
+                                   int32_t nrec, int32_t nt, float *residual, double *loss, char *err, int err_cap);
+
 /* ---- Learned preconditioner (2-D U-Net) kernels: no counterpart in the
  * reference, whose path has no learned component.  Tensors are NHWC views:
  * a device pointer to channel 0 of pixel 0 and a channel stride; bf16 tensors

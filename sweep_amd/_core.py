@@ -71,7 +71,7 @@ EXPORTS = (
     "swp_entry_name", "swp_entry_kind", "swp_entry_available", "swp_core_device", "swp_query",
     "swp_call", "swp_set_stream", "swp_get_stream", "swp_set_timing", "swp_last_kernel_stats",
     "swp_revolve_schedule", "swp_revolve_spine", "swp_record_to_canonical", "swp_record_to_raw",
-    "swp_l2_misfit",
+    "swp_l2_misfit", "swp_l2_misfit_noalloc",
     "swp_nn_conv_fwd", "swp_nn_conv_wgrad_scratch", "swp_nn_conv_wgrad", "swp_nn_conv_flip", "swp_nn_relu_bwd",
     "swp_nn_pool2_fwd", "swp_nn_pool2_bwd_acc", "swp_nn_up2_fwd", "swp_nn_up2_bwd", "swp_nn_pack_input",
     "swp_nn_mse", "swp_nn_adam", "swp_nn_to_bf16", "swp_set_tuning", "swp_get_tuning",
@@ -154,6 +154,9 @@ def _declare(L):
     L.swp_census_enable.argtypes = [_c]
     L.swp_census.restype = _i64
     L.swp_census.argtypes = [ctypes.c_char_p, _i64]
+    L.swp_l2_misfit_noalloc.restype = ctypes.c_int
+    L.swp_l2_misfit_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _i, _i, _i, _i,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]
     L.swp_l2_misfit.restype = ctypes.c_int
     L.swp_l2_misfit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _i, _i, _i, _i,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]

@@ -571,6 +571,12 @@ SWP_CAPI int swp_l2_misfit(const float *pred, const float *obs, const float *mas
     });
 }
 
+SWP_CAPI int swp_l2_misfit_noalloc(const float *pred, const float *obs, const float *mask, int32_t nfield, int32_t B,
+                                   int32_t nrec, int32_t nt, float *residual, double *loss, char *err, int err_cap)
+{
+    return guarded(err, err_cap, [&] { rec::l2_misfit_noalloc(pred, obs, mask, nfield, B, nrec, nt, residual, loss); });
+}
+
 SWP_CAPI int swp_nn_conv_fwd(const uint16_t *in, int32_t N, int32_t H, int32_t W, int32_t cin, int32_t in_stride,
                              const uint16_t *w, int32_t cout, int32_t ks, const float *bias, int32_t relu,
                              void *out, int32_t out_stride, int32_t out_f32, char *err, int err_cap)

@@ -20,6 +20,10 @@ void to_raw(const float *canon, float *raw, int nf, int B, int nrec, int nt);
 int misfit_blocks(int64_t n);
 void l2_misfit(const float *pred, const float *obs, const float *mask, int nf, int B, int nrec, int nt,
                float *residual, double *loss, double *partials);
+// The same, with the partial sums in a static device array of the library
+// (kMisfitPartials doubles per GPU): no workspace argument, no allocation.
+void l2_misfit_noalloc(const float *pred, const float *obs, const float *mask, int nf, int B, int nrec, int nt,
+                       float *residual, double *loss);
 
 }  // namespace rec
 }  // namespace swp

@@ -9,10 +9,19 @@
 // both coalesced.
 #include "records.h"
 
+#include <map>
+#include <mutex>
+
 #include <algorithm>
 
 namespace swp {
 namespace rec {
+
+// Partial sums of l2_misfit_noalloc: library-owned device memory, one copy
+// per GPU, held until the process ends.  Host code never passes this name to
+// a kernel (on the host it is only a stand-in for the symbol); it uses the
+// device address hipGetSymbolAddress returns for the current GPU.
+__device__ __align__(256) double scratch[kMisfitPartials];
 
 namespace {
 
@@ -137,6 +146,25 @@ __global__ void __launch_bounds__(256) misfit_sum_kernel(const double *__restric
     if (threadIdx.x == 0) loss[0] = 0.5 * red[0];
 }
 
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch;   // device ordinal -> that GPU's copy of `scratch`
+
+// The current GPU's address of `scratch`, looked up once per device (with
+// that device current) on the first call there.
+double *scratch_on_current_device()
+{
+    int dev = -1;
+    SWP_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(dev);
+    if (it != g_scratch.end()) return it->second;
+    void *p = nullptr;
+    SWP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SWP_CHECK(p != nullptr, "l2_misfit_noalloc: no device address for the partial-sum array");
+    g_scratch[dev] = static_cast<double *>(p);
+    return static_cast<double *>(p);
+}
+
 }  // namespace
 
 void to_canonical(const float *raw, float *canon, int nf, int B, int nrec, int nt)
@@ -167,6 +195,18 @@ void l2_misfit(const float *pred, const float *obs, const float *mask, int nf, i
     SWP_LAUNCHED();
     hipLaunchKernelGGL(misfit_sum_kernel, dim3(1), dim3(256), 0, stream(), partials, nb, loss);
     SWP_LAUNCHED();
+}
+
+void l2_misfit_noalloc(const float *pred, const float *obs, const float *mask, int nf, int B, int nrec, int nt,
+                       float *residual, double *loss)
+{
+    check_shape(nf, B, nrec, nt);
+    const int64_t n = int64_t(B) * nt * nrec * nf;
+    // the partial-sum count is capped by construction (grid-stride blocks);
+    // the check states the bound the static array is sized for
+    SWP_CHECK(misfit_blocks(n) <= kMisfitPartials, "l2_misfit_noalloc: ", misfit_blocks(n),
+              " partial sums exceed the ", int(kMisfitPartials), " the library's static array holds");
+    l2_misfit(pred, obs, mask, nf, B, nrec, nt, residual, loss, scratch_on_current_device());
 }
 
 }  // namespace rec
