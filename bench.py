@@ -259,28 +259,34 @@ REF_SAMPLE_STEPS = {"acoustic3d": 8, "acoustic2d": 1000, "elastic2d": 300, "acou
 
 
 # ----------------------------------------------------------------- traffic
-# HBM bytes per launch of a kernel class, measured by the two rocprofv3 PMC
-# passes recorded in profiles/ (FETCH_SIZE scaled by 2.000 and WRITE_SIZE by
-# 1.000, the factors a 1 GiB device copy reads in the same runs), for the
-# default shots-per-launch of the workload.  Counters cannot be read inside
-# this process, so the figure is the recorded one.
+# HBM bytes per launch of a kernel class, from the two rocprofv3 PMC passes
+# recorded in profiles/ (FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000,
+# the factors a 1 GiB device copy reads in the same runs), keyed by workload,
+# memory strategy and shots per launch at the default tile settings.
+# Counters cannot be read inside this process, so the figure is the recorded
+# one; any other configuration (another memory mode, shots per launch,
+# --autotune, SWP_TZ* tile overrides) reports None.
 MEASURED_TRAFFIC = {
+    # profiles/acoustic3d_256x512x512_2per_launch_bs.md: the eight region launches of one step
+    ("acoustic3d", "bs", 2): {0: 8.812e9, 1: 10.325e9, 2: 4.770e9},
     # profiles/acoustic2d_384x1152_32shots_bs.md: adj_band 359.9 MB + adj_int 258.2 MB
-    ("acoustic2d", 1): 618.1e6,
-
-    # profiles/acoustic3d_256x512x512_8shots_bs.md: the eight adj3_kernel region launches of one
-    # step at 4 shots per launch (64-plane segments)
-    ("acoustic3d", 1): 20652.0e6,
-
-    # profiles/elastic2d_512x1024_16shots_ckpt.md: vel_kernel 823.7 MB + stress_kernel 1010.6 MB
-    ("elastic2d", 0): 1834.3e6,
+    ("acoustic2d", "bs", 32): {1: 618.1e6},
 }
+TRAFFIC_SOURCE = {
+    ("acoustic3d", "bs", 2): "profiles/acoustic3d_256x512x512_2per_launch_bs.md",
+    ("acoustic2d", "bs", 32): "profiles/acoustic2d_384x1152_32shots_bs.md",
+}
+TILE_ENV = ("SWP_TZ", "SWP_TZ_BAND", "SWP_TZ_RECON", "SWP_TZ3", "SWP_REGION_WAVES", "SWP_KERNELS", "SWP_SHELL3")
 
 
-def measured_traffic(workload, cls, per_launch):
-    if per_launch != WORKLOADS[workload]["per_launch"]:
-        return None
-    return MEASURED_TRAFFIC.get((workload, cls))
+def measured_traffic(workload, memory, cls, per_launch, autotuned):
+    """(bytes, source) recorded for this exact configuration, else (None, why)."""
+    key = (workload, memory or "bs", per_launch)
+    if autotuned or any(os.environ.get(v) for v in TILE_ENV):
+        return None, "tile settings differ from the recorded profile"
+    if key not in MEASURED_TRAFFIC or cls not in MEASURED_TRAFFIC[key]:
+        return None, "no PMC profile recorded for this configuration"
+    return MEASURED_TRAFFIC[key][cls], "recorded: " + TRAFFIC_SOURCE[key]
 
 
 # ----------------------------------------------------------------- main
@@ -408,6 +414,7 @@ def main():
                           gbs=abytes[c] / (ms / n * 1e-3) / 1e9)
     dom = max(per, key=lambda c: per[c]["total_est_ms"])
     ach = per[dom]["gbs"]
+    traffic, traffic_src = measured_traffic(args.workload, args.memory, dom, per_launch, autotuned)
     shots_per_step = per_launch if per_step_launch else nshots
     value = shots_per_step * args.steps / elapsed
     out = {
@@ -426,7 +433,8 @@ def main():
                    **({"autotuned": autotuned} if autotuned else {})},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": None if autotuned else measured_traffic(args.workload, dom, per_launch),
+                     "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": int(abytes[dom]),
                      "avg_launch_ms": per[dom]["avg_ms"],
                      "kernels": {names[c]: {"avg_launch_ms": per[c]["avg_ms"], "GB/s": per[c]["gbs"],
