@@ -150,6 +150,13 @@ def _main():
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--learned-scales", default="1,0.5,0.25",
                     help="comma-separated trial scales of the network's update, tried in order")
+    ap.add_argument("--stage", choices=["all", "train", "compare"], default="all",
+                    help="all: pairs, training and the comparison; train: pairs from --first-model on and "
+                         "training (from --load if given), weights to --save; compare: --load, then plain and "
+                         "learned FWI")
+    ap.add_argument("--first-model", type=int, default=0, help="index of the first training model (seed 100 + k)")
+    ap.add_argument("--load", default=None, help="network weights and Adam state to start from")
+    ap.add_argument("--save", default=None, help="where to write the trained network")
     args = ap.parse_args()
     scales = tuple(float(v) for v in args.learned_scales.split(","))
     dev = torch.device("cuda:0")
@@ -173,22 +180,32 @@ def _main():
                 **kw)
         return f, torch.tensor(synthetic.smoothed(vt_np), device=dev)
 
-    t0 = time.perf_counter()
-    train_models = [synthetic.layered_2d(shape, seed=100 + k) for k in range(args.train_models)]
-    pairs = generate_pairs(lambda vt: make(vt, nshots=args.pair_shots or args.shots), train_models,
-                           args.pair_iterations, dev,
-                           log=lambda k, n: print(json.dumps({"stage": "pairs", "models_done": k, "of": n}),
-                                                  flush=True))
-    torch.cuda.synchronize()
-    print(json.dumps({"stage": "pairs", "count": len(pairs), "seconds": round(time.perf_counter() - t0, 2)}),
-          flush=True)
     net = UNet(dev, levels=4, base=args.base, nsrc=3, seed=0)
-    t0 = time.perf_counter()
-    losses = train(net, pairs, args.train_steps, batch=args.batch, lr=args.lr)
-    torch.cuda.synchronize()
-    print(json.dumps({"stage": "train", "steps": args.train_steps, "loss_first": losses[0],
-                      "loss_last": float(np.mean(losses[-20:])), "seconds": round(time.perf_counter() - t0, 2)}),
-          flush=True)
+    if args.load:
+        net.load(args.load)
+    if args.stage in ("all", "train"):
+        t0 = time.perf_counter()
+        train_models = [synthetic.layered_2d(shape, seed=100 + k)
+                        for k in range(args.first_model, args.first_model + args.train_models)]
+        pairs = generate_pairs(lambda vt: make(vt, nshots=args.pair_shots or args.shots), train_models,
+                               args.pair_iterations, dev,
+                               log=lambda k, n: print(json.dumps({"stage": "pairs", "models_done": k, "of": n,
+                                                                  "seconds": round(time.perf_counter() - t0, 1)}),
+                                                      flush=True))
+        torch.cuda.synchronize()
+        print(json.dumps({"stage": "pairs", "count": len(pairs), "seconds": round(time.perf_counter() - t0, 2)}),
+              flush=True)
+        t0 = time.perf_counter()
+        losses = train(net, pairs, args.train_steps, batch=args.batch, lr=args.lr, seed=args.first_model)
+        torch.cuda.synchronize()
+        print(json.dumps({"stage": "train", "steps": args.train_steps, "loss_first": losses[0],
+                          "loss_last": float(np.mean(losses[-20:])), "seconds": round(time.perf_counter() - t0, 2)}),
+              flush=True)
+        del pairs
+        if args.save:
+            net.save(args.save)
+        if args.stage == "train":
+            return
     held = synthetic.layered_2d(shape, seed=7)
     for name, cls, extra, kw in (("plain", PlainFWI, (), {}),
                                  ("learned", LearnedFWI, (net,), {"learned_scales": scales})):

@@ -113,6 +113,26 @@ class UNet:
     def state(self):
         return {n: (l.w.detach().cpu().clone(), l.b.detach().cpu().clone()) for n, l in self.layers.items()}
 
+    def save(self, path):
+        """Master weights, biases and Adam moments (torch.save of plain tensors)."""
+        out = {"step_count": torch.tensor(self.step_count)}
+        for n, l in self.layers.items():
+            for k in ("w", "b", "mw", "vw", "mb", "vb"):
+                out[f"{n}.{k}"] = getattr(l, k).detach().cpu()
+        torch.save(out, path)
+
+    def load(self, path):
+        """Restore what save() wrote (same architecture); refreshes the bf16 copies."""
+        st = torch.load(path, map_location="cpu", weights_only=True)
+        for n, l in self.layers.items():
+            for k in ("w", "b", "mw", "vw", "mb", "vb"):
+                t = st[f"{n}.{k}"]
+                if tuple(t.shape) != tuple(getattr(l, k).shape):
+                    raise ValueError(f"{n}.{k}: saved {tuple(t.shape)}, network {tuple(getattr(l, k).shape)}")
+                getattr(l, k).copy_(t.to(self.device))
+        self.step_count = int(st["step_count"])
+        self.refresh()
+
     # -- building blocks ---------------------------------------------------
     def _conv(self, lay, x, xs, N, H, W, out, os, relu=True, f32=False):
         _call("swp_nn_conv_fwd", _p(x), N, H, W, lay.cin, xs, _p(lay.wbf), lay.cout, lay.ks, _p(lay.b),
