@@ -11,10 +11,14 @@ There is no fallback: without the built library every entry raises.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libswp_core.so"
+if os.environ.get("SWP_CORE_LIBRARY"):
+    # another build of the same sources (the tests' no-contraction parity build)
+    _LIB_PATH = Path(os.environ["SWP_CORE_LIBRARY"]).resolve()
 ABI_VERSION = 6
 MISFIT_PARTIALS = 1024   # SWP_MISFIT_PARTIALS
 

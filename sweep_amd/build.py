@@ -38,17 +38,29 @@ def sources():
     return sorted(CSRC.glob("*.hip"))
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> Path:
-    """Compile every source and link the shared library; returns its path."""
+# The parity build: the same sources with floating-point contraction off (no
+# a * b + c fused into one rounding) and SWP_EXACT_DIV (stencil sums divided
+# by h^2 or h where the oracle divides, instead of multiplied by a reciprocal),
+# for checking that a record's distance from the oracle, which does neither,
+# comes from these two rounding choices alone.  Tests load it in a subprocess
+# through SWP_CORE_LIBRARY; the product path loads LIB.
+LIB_NOCONTRACT = OUT_DIR / "libswp_core_nocontract.so"
+
+
+def build(verbose: bool = False, jobs: int | None = None, contract: bool = True) -> Path:
+    """Compile every source and link the shared library; returns its path
+    (contract=False: the parity build, LIB_NOCONTRACT)."""
     OUT_DIR.mkdir(exist_ok=True)
-    obj_dir = OUT_DIR / "obj"
+    obj_dir = OUT_DIR / ("obj" if contract else "obj_nocontract")
     obj_dir.mkdir(exist_ok=True)
     cc = hipcc()
     srcs = sources()
+    flags = HIPCC_FLAGS if contract else HIPCC_FLAGS + ["-ffp-contract=off", "-DSWP_EXACT_DIV"]
+    lib = LIB if contract else LIB_NOCONTRACT
 
     def compile_one(src: Path) -> Path:
         obj = obj_dir / (src.stem + ".o")
-        cmd = [cc, *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+        cmd = [cc, *flags, "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -59,15 +71,15 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     jobs = jobs or min(len(srcs), 8)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, srcs))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":
-    path = build(verbose="-v" in sys.argv)
+    path = build(verbose="-v" in sys.argv, contract="--no-contract" not in sys.argv)
     print(path)

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "fd.h"
 
 namespace swp {
 namespace a2d {
@@ -23,7 +24,12 @@ struct Geo {
     int64_t ns;                // strip cells per shot per step
     int sbf16;                 // strips stored as bfloat16 (SWP_FLAG_BF16_STRIPS)
     float dt, dx, dz;
-    float inv_dx2, inv_dz2, inv_dx, inv_dz, dt2, inv_dt2;
+    // 1/h^2, 1/h and 1/dt^2 with h itself (fd::Scale): the forward and
+    // reconstruction stencils and the u_tt of imaging scale through
+    // fd::scaled; the adjoint multiplies by .inv
+    fd::Scale inv_dx2, inv_dz2, inv_dx, inv_dz;
+    float dt2;
+    fd::Scale inv_dt2;
 };
 
 // FD weights: the fixed-order forward constants and the caller's tables.

@@ -48,12 +48,12 @@ Geo make_geo(const swp_problem &p)
     g.nzp = g.Z1 - g.Z0;
     g.ns = 2LL * g.W * g.nxp + 2LL * g.W * g.nzp;
     g.sbf16 = (p.flags & SWP_FLAG_BF16_STRIPS) != 0;
-    g.inv_dx2 = 1.0f / (g.dx * g.dx);
-    g.inv_dz2 = 1.0f / (g.dz * g.dz);
-    g.inv_dx = 1.0f / g.dx;
-    g.inv_dz = 1.0f / g.dz;
+    g.inv_dx2 = fd::scale_of(g.dx * g.dx);
+    g.inv_dz2 = fd::scale_of(g.dz * g.dz);
+    g.inv_dx = fd::scale_of(g.dx);
+    g.inv_dz = fd::scale_of(g.dz);
     g.dt2 = g.dt * g.dt;
-    g.inv_dt2 = 1.0f / (g.dt * g.dt);
+    g.inv_dt2 = fd::scale_of(g.dt * g.dt);
     return g;
 }
 
@@ -117,42 +117,42 @@ Coef make_coef(const swp_problem &p, const float *lap_host, const float *grad_ho
 // fixed orders in difference form, runtime order from the table.
 template <int MS>
 __device__ __forceinline__ float lap_fwd(const Coef &c, const float *__restrict__ u, int64_t i,
-                                         int64_t s, float inv_h2, int M)
+                                         int64_t s, fd::Scale inv_h2, int M)
 {
     const float u0 = u[i];
     if constexpr (MS == 1) {
-        return ((u[i - s] - u0) + (u[i + s] - u0)) * inv_h2;
+        return fd::scaled((u[i - s] - u0) + (u[i + s] - u0), inv_h2);
     } else if constexpr (MS == 2) {
-        return (c.slap[0] * (u[i - s] + u[i + s]) + c.slap[1] * (u[i - 2 * s] + u[i + 2 * s]) -
-                c.slap_c0 * u0) * inv_h2;
+        return fd::scaled(c.slap[0] * (u[i - s] + u[i + s]) + c.slap[1] * (u[i - 2 * s] + u[i + 2 * s]) -
+                          c.slap_c0 * u0, inv_h2);
     } else if constexpr (MS == 3 || MS == 4) {
         float acc = 0.f;
 #pragma unroll
         for (int k = 1; k <= MS; ++k)
             acc += c.slap[k - 1] * ((u[i - k * s] - u0) + (u[i + k * s] - u0));
-        return acc * inv_h2;
+        return fd::scaled(acc, inv_h2);
     } else {
         float acc = 0.f;
         for (int k = 1; k <= M; ++k) acc += c.lap[k] * ((u[i + k * s] - u0) + (u[i - k * s] - u0));
-        return acc * inv_h2;
+        return fd::scaled(acc, inv_h2);
     }
 }
 
 // Centred first derivative of f(k), k = -M..M (gradient.cuh:47-88).
 template <int MS, class F>
-__device__ __forceinline__ float d1(const Coef &c, F f, float inv_h, int M)
+__device__ __forceinline__ float d1(const Coef &c, F f, fd::Scale inv_h, int M)
 {
     if constexpr (MS == 1) {
-        return (f(1) - f(-1)) * (0.5f * inv_h);
+        return fd::scaled_half(f(1) - f(-1), inv_h);
     } else if constexpr (MS >= 2 && MS <= 4) {
         float acc = 0.f;
 #pragma unroll
         for (int k = 1; k <= MS; ++k) acc += c.sgrad[k - 1] * (f(k) - f(-k));
-        return acc * inv_h;
+        return fd::scaled(acc, inv_h);
     } else {
         float acc = 0.f;
         for (int k = 1; k <= M; ++k) acc += c.grad[k] * (f(k) - f(-k));
-        return acc * inv_h;
+        return fd::scaled(acc, inv_h);
     }
 }
 
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
                 lapz += c.lap[k] * ((GW(i + k * nx) - gw) + (GW(i - k * nx) - gw));
             }
         }
-        f.l_next[off + i] = 2.0f * gun - f.l_prev[off + i] + lapx * g.inv_dx2 + lapz * g.inv_dz2;
+        f.l_next[off + i] = 2.0f * gun - f.l_prev[off + i] + lapx * g.inv_dx2.inv + lapz * g.inv_dz2.inv;
         return;
     }
     const float *__restrict__ psix = f.psix + off;
@@ -312,12 +312,12 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
             Dqz += c.grad[k] * (tp - tm);
         }
     }
-    Lx *= g.inv_dx2;
-    Dgx *= g.inv_dx;
-    Dqx *= g.inv_dx;
-    Lz *= g.inv_dz2;
-    Dgz *= g.inv_dz;
-    Dqz *= g.inv_dz;
+    Lx *= g.inv_dx2.inv;
+    Dgx *= g.inv_dx.inv;
+    Dqx *= g.inv_dx.inv;
+    Lz *= g.inv_dz2.inv;
+    Dgz *= g.inv_dz.inv;
+    Dqz *= g.inv_dz.inv;
     f.l_next[off + i] = 2.0f * gun - f.l_prev[off + i] + (Lx + Lz - Dgx - Dgz);
     f.psixo[off + i] = ax_ * (psix[i] - Dqx);
     f.psizo[off + i] = az_ * (psiz[i] - Dqz);
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(256) recon_kernel(Geo g, Coef c, const float *
     const float unx = 2.0f * u0 - up + (v * v) * g.dt * g.dt * w_sum;
     u_next[off + i] = unx;
     if (grad && ix >= g.X0 + g.W && ix < g.X1 - g.W && iz >= g.Z0 + g.W && iz < g.Z1 - g.W) {
-        const float u_tt = (unx - 2 * u0 + up) * g.inv_dt2;
+        const float u_tt = fd::scaled(unx - 2 * u0 + up, g.inv_dt2);
         grad[off + i] += 2.f * g.dt * g.dt * u_tt * adj[off + i] / v;
     }
 }
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(256) strip_restore_kernel(
     const int64_t i = int64_t(iz) * g.px + ix;
     u_next[off + i] = val;
     if (grad) {
-        const float u_tt = (val - 2 * u_now[off + i] + u_prev[off + i]) * g.inv_dt2;
+        const float u_tt = fd::scaled(val - 2 * u_now[off + i] + u_prev[off + i], g.inv_dt2);
         grad[off + i] += 2.f * g.dt * g.dt * u_tt * adj[off + i] / vpb[int64_t(b) * g.mstride + i];
     }
 }
@@ -426,7 +426,7 @@ __global__ void __launch_bounds__(256) image_kernel(Geo g, const float *__restri
     if (S) {
         float u_tt;
         if (!fnow) u_tt = img[off + i];
-        else u_tt = (fnow[off + i] - 2 * fprev[off + i] + img[off + i]) * g.inv_dt2;
+        else u_tt = fd::scaled(fnow[off + i] - 2 * fprev[off + i] + img[off + i], g.inv_dt2);
         S[off + i] += u_tt * u_tt;
     }
 }

@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
         if (active) {
             const float gun = ln[rz + x];
             if constexpr (IMG) grad[off + rz + x] += 2.f * g.dt * g.dt * im * gun / vp[rz + x];
-            lnx[rz + x] = 2.0f * gun - lpv + lapx * g.inv_dx2 + lapz * g.inv_dz2;
+            lnx[rz + x] = 2.0f * gun - lpv + lapx * g.inv_dx2.inv + lapz * g.inv_dz2.inv;
         }
 #pragma unroll
         for (int k = 0; k < 2 * M; ++k) q[k] = q[k + 1];
@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
                 lapx += c.lap[k] * ((rg[M + lane + k] - gw) + (rg[M + lane - k] - gw));
                 lapz += c.lap[k] * ((qg[M + k] - gw) + (qg[M - k] - gw));
             }
-            if (active) lnx[rz + x] = 2.0f * gun - lpv + lapx * g.inv_dx2 + lapz * g.inv_dz2;
+            if (active) lnx[rz + x] = 2.0f * gun - lpv + lapx * g.inv_dx2.inv + lapz * g.inv_dz2.inv;
         } else {
             float Lx = 0.f, Dgx = 0.f, Dqx = 0.f, Lz = 0.f, Dgz = 0.f, Dqz = 0.f;
             if (wave_x) {
@@ -494,8 +494,8 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
                     Dqz += c.grad[k] * (qg[M + k] - qg[M - k]);
                 }
             }
-            Lx *= g.inv_dx2; Dgx *= g.inv_dx; Dqx *= g.inv_dx;
-            Lz *= g.inv_dz2; Dgz *= g.inv_dz; Dqz *= g.inv_dz;
+            Lx *= g.inv_dx2.inv; Dgx *= g.inv_dx.inv; Dqx *= g.inv_dx.inv;
+            Lz *= g.inv_dz2.inv; Dgz *= g.inv_dz.inv; Dqz *= g.inv_dz.inv;
             if (active) {
                 lnx[rz + x] = 2.0f * gun - lpv + (Lx + Lz - Dgx - Dgz);
                 if (wave_x) {
@@ -580,7 +580,7 @@ __global__ void __launch_bounds__(64) recon_zm_kernel(Geo g, Coef c, const float
         if (active) {
             u_next[off + rz + x] = unx;
             if (img_row && ximg) {
-                const float u_tt = (unx - 2 * u0 + upv) * g.inv_dt2;
+                const float u_tt = fd::scaled(unx - 2 * u0 + upv, g.inv_dt2);
                 grad[off + rz + x] += 2.f * dt * dt * u_tt * a / v;
             }
         }

@@ -24,7 +24,12 @@ struct Geo {
     int64_t ns;                         // strip cells per shot per step (six faces)
     int sbf16;                          // strips stored as bfloat16 (SWP_FLAG_BF16_STRIPS)
     float dt, dx, dy, dz;
-    float inv_dx2, inv_dy2, inv_dz2, inv_dx, inv_dy, inv_dz, dt2, inv_dt2;
+    // 1/h^2, 1/h and 1/dt^2 with h itself (fd::Scale): the forward and
+    // reconstruction stencils, the profile derivative and the u_tt of imaging
+    // scale through fd::scaled; the adjoint's stencil sums multiply by .inv
+    fd::Scale inv_dx2, inv_dy2, inv_dz2, inv_dx, inv_dy, inv_dz;
+    float dt2;
+    fd::Scale inv_dt2;
 };
 
 // CPML profiles per axis (a, b, db on the runtime length of the axis).

@@ -64,14 +64,14 @@ Geo make_geo(const swp_problem &p)
     g.dx = p.dx;
     g.dy = p.dy;
     g.dz = p.dz;
-    g.inv_dx2 = 1.0f / (g.dx * g.dx);
-    g.inv_dy2 = 1.0f / (g.dy * g.dy);
-    g.inv_dz2 = 1.0f / (g.dz * g.dz);
-    g.inv_dx = 1.0f / g.dx;
-    g.inv_dy = 1.0f / g.dy;
-    g.inv_dz = 1.0f / g.dz;
+    g.inv_dx2 = fd::scale_of(g.dx * g.dx);
+    g.inv_dy2 = fd::scale_of(g.dy * g.dy);
+    g.inv_dz2 = fd::scale_of(g.dz * g.dz);
+    g.inv_dx = fd::scale_of(g.dx);
+    g.inv_dy = fd::scale_of(g.dy);
+    g.inv_dz = fd::scale_of(g.dz);
     g.dt2 = g.dt * g.dt;
-    g.inv_dt2 = 1.0f / (g.dt * g.dt);
+    g.inv_dt2 = fd::scale_of(g.dt * g.dt);
     return g;
 }
 
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(256) fwd_kernel(Geo g, Coef c, Prof pr, StepFi
     const int64_t S[3] = {1, g.px, g.sxy};
     const int pos[3] = {ix, iy, iz};
     const int lo[3] = {g.X0, g.Y0, g.Z0}, hi[3] = {g.X1, g.Y1, g.Z1};
-    const float ih2[3] = {g.inv_dx2, g.inv_dy2, g.inv_dz2}, ih[3] = {g.inv_dx, g.inv_dy, g.inv_dz};
+    const fd::Scale ih2[3] = {g.inv_dx2, g.inv_dy2, g.inv_dz2}, ih[3] = {g.inv_dx, g.inv_dy, g.inv_dz};
     float L[3];
     bool inb[3];
     bool any = false;
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
     const int64_t S[3] = {1, g.px, g.sxy};
     const int pos[3] = {ix, iy, iz};
     const int lo[3] = {g.X0, g.Y0, g.Z0}, hi[3] = {g.X1, g.Y1, g.Z1};
-    const float ih2[3] = {g.inv_dx2, g.inv_dy2, g.inv_dz2}, ih[3] = {g.inv_dx, g.inv_dy, g.inv_dz};
+    const fd::Scale ih2[3] = {g.inv_dx2, g.inv_dy2, g.inv_dz2}, ih[3] = {g.inv_dx, g.inv_dy, g.inv_dz};
     bool in[3];
     bool any = false;
 #pragma unroll
@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
             }
             lp[a] = acc;
         }
-        f.l_next[off + i] = 2.0f * gun - f.l_prev[off + i] + lp[0] * g.inv_dx2 + lp[1] * g.inv_dy2 + lp[2] * g.inv_dz2;
+        f.l_next[off + i] = 2.0f * gun - f.l_prev[off + i] + lp[0] * g.inv_dx2.inv + lp[1] * g.inv_dy2.inv + lp[2] * g.inv_dz2.inv;
         return;
     }
     float Lt[3], Dg[3];
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
         if (!in[a]) {
             float L = 0.f;
             for (int k = 1; k <= M; ++k) L += c.lap[k] * ((GW(i + k * s) - gw) + (GW(i - k * s) - gw));
-            Lt[a] = L * ih2[a];
+            Lt[a] = L * ih2[a].inv;
             Dg[a] = 0.f;
             continue;
         }
@@ -234,9 +234,9 @@ __global__ void __launch_bounds__(256) adj_kernel(Geo g, Coef c, Prof pr, AdjFie
             Dgg += c.grad[k] * ((bp * psi[np] + Da[cp] * tp) - (bm * psi[nm] + Da[cm] * tm));
             Dq += c.grad[k] * (Aa[cp] * tp - Aa[cm] * tm);
         }
-        Lt[a] = L * ih2[a];
-        Dg[a] = Dgg * ih[a];
-        Dq *= ih[a];
+        Lt[a] = L * ih2[a].inv;
+        Dg[a] = Dgg * ih[a].inv;
+        Dq *= ih[a].inv;
         f.psio[a][off + i] = a_ * psi[i] + da * gt0 - Dq;
         f.zetao[a][off + i] = a_ * (zeta[i] + gw);
     }
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(256) recon_kernel(Geo g, Coef c, const float *
     u_next[off + i] = unx;
     if (grad && ix >= g.X0 + g.W && ix < g.X1 - g.W && iy >= g.Y0 + g.W && iy < g.Y1 - g.W &&
         iz >= g.Z0 + g.W && iz < g.Z1 - g.W) {
-        const float u_tt = (unx - 2 * u0 + up) * g.inv_dt2;
+        const float u_tt = fd::scaled(unx - 2 * u0 + up, g.inv_dt2);
         grad[off + i] += 2.f * g.dt * g.dt * u_tt * adj[off + i] / v;
     }
 }
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(256) strip_restore_kernel(
     const int64_t i = (int64_t(iz) * g.ny + iy) * g.px + ix;
     u_next[off + i] = val;
     if (grad) {
-        const float u_tt = (val - 2 * u_now[off + i] + u_prev[off + i]) * g.inv_dt2;
+        const float u_tt = fd::scaled(val - 2 * u_now[off + i] + u_prev[off + i], g.inv_dt2);
         grad[off + i] += 2.f * g.dt * g.dt * u_tt * adj[off + i] / vpb[int64_t(b) * g.mstride + i];
     }
 }
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(256) image_kernel(Geo g, const float *__restri
     if (S) {
         float u_tt;
         if (!fnow) u_tt = img[off + i];
-        else u_tt = (fnow[off + i] - 2 * fprev[off + i] + img[off + i]) * g.inv_dt2;
+        else u_tt = fd::scaled(fnow[off + i] - 2 * fprev[off + i] + img[off + i], g.inv_dt2);
         S[off + i] += u_tt * u_tt;
     }
 }

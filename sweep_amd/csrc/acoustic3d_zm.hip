@@ -783,7 +783,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
                     ly += c.lap[t] * ((Gw[t * LW] - gw) + (Gw[-t * LW] - gw));
                     lz += c.lap[t] * ((zq(qg[r], t) - gw) + (zq(qg[r], -t) - gw));
                 }
-                out = 2.0f * gun - clp[r] + lx * g.inv_dx2 + ly * g.inv_dy2 + lz * g.inv_dz2;
+                out = 2.0f * gun - clp[r] + lx * g.inv_dx2.inv + ly * g.inv_dy2.inv + lz * g.inv_dz2.inv;
             } else {
                 float Lt[3], Dg[3];
                 if constexpr (BX) {
@@ -797,16 +797,16 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
                         Dgg += c.grad[t] * (Vx[t] - Vx[-t]);
                         Dq += c.grad[t] * (Wx[t] - Wx[-t]);
                     }
-                    Lt[0] = Ls * g.inv_dx2;
-                    Dg[0] = Dgg * g.inv_dx;
-                    Dq *= g.inv_dx;
+                    Lt[0] = Ls * g.inv_dx2.inv;
+                    Dg[0] = Dgg * g.inv_dx.inv;
+                    Dq *= g.inv_dx.inv;
                     pox.st_nt(ax * cpx[r] + dax * gt0 - Dq, col[r], so);
                     zox.st_nt(ax * (czx[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
                     for (int t = 1; t <= M; ++t) Ls += c.lap[t] * ((Gw[t] - gw) + (Gw[-t] - gw));
-                    Lt[0] = Ls * g.inv_dx2;
+                    Lt[0] = Ls * g.inv_dx2.inv;
                     Dg[0] = 0.f;
                 }
                 if constexpr (BY) {
@@ -820,16 +820,16 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
                         Dgg += c.grad[t] * (Vy[t * LW] - Vy[-t * LW]);
                         Dq += c.grad[t] * (Wy[t * LW] - Wy[-t * LW]);
                     }
-                    Lt[1] = Ls * g.inv_dy2;
-                    Dg[1] = Dgg * g.inv_dy;
-                    Dq *= g.inv_dy;
+                    Lt[1] = Ls * g.inv_dy2.inv;
+                    Dg[1] = Dgg * g.inv_dy.inv;
+                    Dq *= g.inv_dy.inv;
                     poy.st_nt(ay[r] * cpy[r] + day[r] * gt0 - Dq, col[r], so);
                     zoy.st_nt(ay[r] * (czy[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
                     for (int t = 1; t <= M; ++t) Ls += c.lap[t] * ((Gw[t * LW] - gw) + (Gw[-t * LW] - gw));
-                    Lt[1] = Ls * g.inv_dy2;
+                    Lt[1] = Ls * g.inv_dy2.inv;
                     Dg[1] = 0.f;
                 }
                 if constexpr (BZ) {
@@ -842,16 +842,16 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW
                         Dgg += c.grad[t] * (zq(qv[r], t) - zq(qv[r], -t));
                         Dq += c.grad[t] * (zq(qw[r], t) - zq(qw[r], -t));
                     }
-                    Lt[2] = Ls * g.inv_dz2;
-                    Dg[2] = Dgg * g.inv_dz;
-                    Dq *= g.inv_dz;
+                    Lt[2] = Ls * g.inv_dz2.inv;
+                    Dg[2] = Dgg * g.inv_dz.inv;
+                    Dq *= g.inv_dz.inv;
                     poz.st_nt(az * cpz[r] + daz * gt0 - Dq, col[r], so);
                     zoz.st_nt(az * (czz[r] + gw), col[r], so);
                 } else {
                     float Ls = 0.f;
 #pragma unroll
                     for (int t = 1; t <= M; ++t) Ls += c.lap[t] * ((zq(qg[r], t) - gw) + (zq(qg[r], -t) - gw));
-                    Lt[2] = Ls * g.inv_dz2;
+                    Lt[2] = Ls * g.inv_dz2.inv;
                     Dg[2] = 0.f;
                 }
                 out = 2.0f * gun - clp[r] + (Lt[0] + Lt[1] + Lt[2] - Dg[0] - Dg[1] - Dg[2]);
@@ -985,7 +985,7 @@ __global__ void __launch_bounds__(64 * NW) recon_zm_kernel(Geo g, Coef c, const 
             const float unxt = 2.0f * u0 - cup[r] + (v * v) * dt * dt * w_sum;
             unx.st(unxt, col[r], so);
             if (grad && img_x && img_y[r] && img_z) {
-                const float u_tt = (unxt - 2 * u0 + cup[r]) * g.inv_dt2;
+                const float u_tt = fd::scaled(unxt - 2 * u0 + cup[r], g.inv_dt2);
                 gr.st(cg[r] + 2.f * dt * dt * u_tt * ca[r] / v, col[r], so);
             }
         }

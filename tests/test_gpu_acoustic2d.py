@@ -140,6 +140,8 @@ def _raw(canon):
 # FMA contraction moves it by 4.1e-5 (test_oracle_acoustic2d.py::
 # test_fma_contraction_sensitivity_of_c1), and the GPU contracts as the
 # reference's --use_fast_math CUDA core does; its bar is that measured floor.
+# test_gpu_nocontract.py runs c1 on the same sources built with contraction
+# off and holds that build to the 1e-5 bar.
 RECORD_BAR = {("c1_order4", "obs"): 6e-5}
 
 

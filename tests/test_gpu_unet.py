@@ -233,3 +233,28 @@ def test_unet_training_lowers_the_loss():
     losses = [net.train_step(x, target, lr=3e-3).item() for _ in range(20)]
     print(f"[unet] loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
     assert losses[-1] < 0.5 * losses[0]
+
+
+def test_save_load_resumes_training_exactly(tmp_path):
+    """save() then load() into a fresh network of another seed restores the
+    weights, biases and Adam moments: the next training step of both networks
+    gives the same loss and the same forward output, bit for bit."""
+    print("[unet] save / load round trip", flush=True)
+    g = torch.Generator().manual_seed(5)
+    N, H, W = 2, 32, 64
+    planes = [torch.randn((N, H, W), generator=g).to(DEV) for _ in range(3)]
+    a = UNet(DEV, levels=3, base=16, nsrc=3, seed=4)
+    x, _ = a.input(planes, [1.0, 1.0, 1.0])
+    target = (0.5 * planes[0] - 0.25 * planes[2]).contiguous()
+    for _ in range(3):
+        a.train_step(x, target, lr=3e-3)
+    a.save(tmp_path / "net.pt")
+    b = UNet(DEV, levels=3, base=16, nsrc=3, seed=9)
+    assert not torch.equal(b.forward(x), a.forward(x))
+    b.load(tmp_path / "net.pt")
+    assert b.step_count == a.step_count
+    la, lb = a.train_step(x, target, lr=3e-3).item(), b.train_step(x, target, lr=3e-3).item()
+    assert la == lb
+    assert torch.equal(a.forward(x), b.forward(x))
+    with pytest.raises(ValueError):
+        UNet(DEV, levels=3, base=8, nsrc=3, seed=0).load(tmp_path / "net.pt")
