@@ -173,3 +173,47 @@ def test_refusals():
     ms = [torch.tensor(m, device=dev, requires_grad=True) for m in c["models"]]
     with pytest.raises(NotImplementedError):
         solver(w, c["src"], c["rec"], models=ms)
+
+
+def test_workload_grid_matches_oracle():
+    """One shot on the 512 x 1024 elastic workload grid (runtime 620 x 1132,
+    order 8, CPML 50; the workload's layered vp, vs and rho; a mid-surface
+    source and 400 receivers two cells apart), nt 600: full storage against
+    the oracle (records 1e-5, gradients 1e-4), checkpointing bit-identical to
+    full storage, boundary saving within 1e-4 of it.  The receivers take
+    distinct cells: receivers sharing a cell take their adjoint sources
+    through float atomics whose order, and so whose rounding, varies from run
+    to run (as in the reference's add_source, common.cu:11-37), which would
+    leave nothing bit-identical to compare."""
+    dev = _dev()
+    from sweep_amd import synthetic
+    shape, nt, dh, dt = (512, 1024), 600, 5.0, 5e-4
+    vp = synthetic.layered_2d(shape, seed=6, vmin=1800.0, vmax=4500.0)
+    rng = np.random.default_rng(13)
+    vs = (vp / 1.8 * (1.0 + 0.1 * rng.uniform(-1, 1, shape))).astype(np.float32)
+    rho = (310.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    src = np.array([[512, 2]])
+    rec = np.stack([np.arange(112, 912, 2), np.full(400, 2)], -1)[None]
+    wav = synthetic.ricker(np.arange(nt) * dt - 1.5 / 12.0, 12.0).astype(np.float32)
+    st, rt = ("vz",), ("vx", "vz")
+    res = rng.standard_normal((2, 1, rec.shape[1], nt)).astype(np.float32)
+    c = dict(order=8, shape=shape, dh=dh, dt=dt, abcn=50, nt=nt, B=1, models=[vp, vs, rho], wav=wav, src=src,
+             rec=rec)
+    out = {}
+    for mode in ("full", "ckpt", "bs"):
+        print(f"[c3grid] shape={shape} nt={nt} one shot mode={mode}", flush=True)
+        out[mode] = _gpu_run(8, shape, dh, dt, 50, mode, c["models"], wav, src, rec, res, dev, list(st), list(rt))
+        torch.cuda.empty_cache()
+    print("[c3grid] oracle", flush=True)
+    ref = _oracle_run(c, st, rt, res)
+    errs = {k: rel_l2(out["full"][k], ref[k]) for k in ("rec", "grad_vp", "grad_vs", "grad_rho")}
+    ebs = {k: rel_l2(out["bs"][k], out["full"][k]) for k in ("grad_vp", "grad_vs", "grad_rho")}
+    print(f"[c3grid] full vs oracle {errs}; bs vs full {ebs}", flush=True)
+    assert np.abs(ref["grad_vp"]).max() > 0
+    assert errs["rec"] <= 1e-5
+    for k in ("grad_vp", "grad_vs", "grad_rho"):
+        assert errs[k] <= 1e-4, k
+        assert ebs[k] <= 1e-4, k
+    for k in out["full"]:
+        assert np.array_equal(out["full"][k], out["ckpt"][k]), k
+    assert np.array_equal(out["full"]["rec"], out["bs"]["rec"])
