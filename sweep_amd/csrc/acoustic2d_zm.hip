@@ -29,10 +29,12 @@ namespace a2d {
 
 #define WAVE_SYNC() __syncthreads()
 
-// This block's strip and row segment [zb, ze).
-__device__ __forceinline__ void block_coords(const BlockSet &bs, int &strip, int &zb, int &ze)
+// Block r's strip and row segment [zb, ze).  The kernels take r and the shot
+// from the XCD-local numbering (xcd_block): each XCD marches a contiguous run
+// of strips and segments of whole shots, so the x-halo columns and segment
+// prologue rows a block re-reads were fetched into the same L2.
+__device__ __forceinline__ void block_coords(const BlockSet &bs, int r, int &strip, int &zb, int &ze)
 {
-    int r = blockIdx.x;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const Rect &q = bs.r[k];
@@ -78,7 +80,8 @@ __global__ void __launch_bounds__(64) fwd_int_kernel(Geo g, Coef c, StepFields f
 {
     __shared__ float row[64 + 2 * M];
     int strip, zb, ze;
-    block_coords(bs, strip, zb, ze);
+    const BlockXYZ blk = xcd_block();
+    block_coords(bs, blk.x, strip, zb, ze);
     if (ze <= zb) return;
     const int lane = threadIdx.x;
     const int xb = bs.x0 + strip * 64;
@@ -86,11 +89,11 @@ __global__ void __launch_bounds__(64) fwd_int_kernel(Geo g, Coef c, StepFields f
     const int nx = g.nx, px = g.px;
     const int xc = min(x, nx - 1);
     const bool active = x >= M && x < nx - M;
-    const int64_t off = int64_t(blockIdx.z) * g.n;
+    const int64_t off = int64_t(blk.z) * g.n;
     const float *__restrict__ un = f.u_now + off;
     const float *__restrict__ up = f.u_prev + off;
     float *__restrict__ unx = f.u_next + off;
-    const float *__restrict__ vp = f.vp + int64_t(blockIdx.z) * g.mstride;
+    const float *__restrict__ vp = f.vp + int64_t(blk.z) * g.mstride;
     const Halo<M> h(lane, xb, nx);
     const float dt = g.dt;
 
@@ -138,7 +141,8 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
     __shared__ float row[64 + 2 * M];
     __shared__ float prow[64 + 2 * M];
     int strip, zb, ze;
-    block_coords(bs, strip, zb, ze);
+    const BlockXYZ blk = xcd_block();
+    block_coords(bs, blk.x, strip, zb, ze);
     if (ze <= zb) return;
     const int lane = threadIdx.x;
     const int xb = bs.x0 + strip * 64;
@@ -146,7 +150,7 @@ __global__ void __launch_bounds__(64) fwd_band_kernel(Geo g, Coef c, Prof pr, St
     const int nx = g.nx, px = g.px, nz = g.nz;
     const int xc = min(x, nx - 1);
     const bool active = x >= M && x < nx - M;
-    const int b = blockIdx.z;
+    const int b = blk.z;
     const int64_t off = int64_t(b) * g.n;
     const float *__restrict__ un = f.u_now + off;
     const float *__restrict__ up = f.u_prev + off;
@@ -281,7 +285,8 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
 {
     __shared__ float row[64 + 2 * M];
     int strip, zb, ze;
-    block_coords(bs, strip, zb, ze);
+    const BlockXYZ blk = xcd_block();
+    block_coords(bs, blk.x, strip, zb, ze);
     if (ze <= zb) return;
     const int lane = threadIdx.x;
     const int xb = bs.x0 + strip * 64;
@@ -289,7 +294,7 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
     const int nx = g.nx, px = g.px;
     const int xc = min(x, nx - 1);
     const bool active = x >= M && x < nx - M;
-    const int b = blockIdx.z;
+    const int b = blk.z;
     const int64_t off = int64_t(b) * g.n;
     const float *__restrict__ ln = f.l_now + off;
     const float *__restrict__ lp = f.l_prev + off;
@@ -352,7 +357,8 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
     //   T = b zeta + ((1+b) c) lambda,  A = (1+b) T,  G = b psi + db T
     __shared__ float rg[64 + 2 * M], rT[64 + 2 * M], rA[64 + 2 * M], rG[64 + 2 * M];
     int strip, zb, ze;
-    block_coords(bs, strip, zb, ze);
+    const BlockXYZ blk = xcd_block();
+    block_coords(bs, blk.x, strip, zb, ze);
     if (ze <= zb) return;
     const int lane = threadIdx.x;
     const int xb = bs.x0 + strip * 64;
@@ -360,7 +366,7 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
     const int nx = g.nx, px = g.px;
     const int xc = min(x, nx - 1);
     const bool active = x >= M && x < nx - M;
-    const int b = blockIdx.z;
+    const int b = blk.z;
     const int64_t off = int64_t(b) * g.n;
     const float *__restrict__ ln = f.l_now + off;
     const float *__restrict__ lp = f.l_prev + off;
@@ -534,14 +540,15 @@ __global__ void __launch_bounds__(64) recon_zm_kernel(Geo g, Coef c, const float
 {
     __shared__ float row[64 + 2 * M];
     const int lane = threadIdx.x;
-    const int xb = x_lo - x_lo % 32 + int(blockIdx.x) * 64;   // strips on 128-byte lines
+    const BlockXYZ blk = xcd_block();
+    const int xb = x_lo - x_lo % 32 + blk.x * 64;   // strips on 128-byte lines
     const int x = xb + lane;
     const int nx = g.nx, px = g.px;
     const int xc = min(x, nx - 1);
     const bool active = x >= x_lo && x < x_hi;
-    const int zb = z_lo + int(blockIdx.y) * tz;
+    const int zb = z_lo + blk.y * tz;
     const int ze = min(zb + tz, z_hi);
-    const int b = blockIdx.z;
+    const int b = blk.z;
     const int64_t off = int64_t(b) * g.n;
     const float *__restrict__ un = u_now + off;
     const float *__restrict__ up = u_prev + off;

@@ -54,13 +54,6 @@ struct ZSet {
     int per_shot;   // blocks per shot
 };
 
-__device__ __forceinline__ int xcd_remap(int f, int total)
-{
-    const int q = total / 8, r = total % 8;
-    const int xcd = f % 8, local = f / 8;
-    return xcd * q + min(xcd, r) + local;
-}
-
 // Calls f(integral_constant<int, I>) for I = B..N-1 in order while it returns true.
 template <int I, int N, class F>
 __device__ __forceinline__ void static_for(F &&f)
@@ -111,7 +104,7 @@ template <int TY>
 __device__ __forceinline__ bool zcoords(const ZSet &s, int &b, int &k, int &xo, int &yo, int &zb, int &ze)
 {
     const int total = int(gridDim.x);
-    const int f = xcd_remap(int(blockIdx.x), total);
+    const int f = int(xcd_local_id(blockIdx.x, unsigned(total)));
     b = f / s.per_shot;
     int t = f % s.per_shot;
     for (k = 0; k < s.n; ++k) {

@@ -31,6 +31,32 @@ void launch_points(const PointArgs &a, int B);
 // to nearest, ties to even, a NaN kept a quiet NaN (the rounding of the
 // reference's __float2bfloat16).  A strip pointer is typed float* either way;
 // strip_at / strip_put / strip_get count elements of the stored kind.
+// XCD-local block numbering.  The dispatcher hands workgroups to the eight
+// XCDs round-robin in launch order (workgroup i to XCD i mod 8) and each XCD
+// has its own L2, so with the plain numbering a tile's row neighbours -- whose
+// halo rows it re-reads -- sit in other L2s.  Renumbered, XCD k takes the k-th
+// contiguous eighth of the grid (row-major), and neighbouring tiles share an
+// L2.  The map is a bijection of [0, n): the first n mod 8 XCDs take one block
+// more.
+__device__ __forceinline__ uint32_t xcd_local_id(uint32_t pid, uint32_t n)
+{
+    const uint32_t xcd = pid % 8u, k = pid / 8u, q = n / 8u, r = n % 8u;
+    return xcd < r ? xcd * (q + 1u) + k : r * (q + 1u) + (xcd - r) * q + k;
+}
+
+struct BlockXYZ {
+    int x, y, z;
+};
+
+// This block's (x, y, z) in its grid under the XCD-local numbering.
+__device__ __forceinline__ BlockXYZ xcd_block()
+{
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    const uint32_t n = gx * gy * gridDim.z;
+    const uint32_t l = xcd_local_id(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), n);
+    return {int(l % gx), int((l / gx) % gy), int(l / (gx * gy))};
+}
+
 __host__ __device__ inline uint16_t bf16_rne(float f)
 {
     const uint32_t u = __builtin_bit_cast(uint32_t, f);

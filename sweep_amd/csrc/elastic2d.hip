@@ -154,9 +154,10 @@ __global__ void __launch_bounds__(256) stress_nopml_kernel(Geo g, SCoef c, Field
                                                            const float *__restrict__ mub)
 {
     const int M = MS > 0 ? MS : g.M;
-    const int ix = g.X0 + 1 + int(blockIdx.x) * 64 + int(threadIdx.x);
-    const int iz = g.Z0 + 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
-    const int b = blockIdx.z;
+    const BlockXYZ blk = xcd_block();
+    const int ix = g.X0 + 1 + blk.x * 64 + int(threadIdx.x);
+    const int iz = g.Z0 + 1 + blk.y * 4 + int(threadIdx.y);
+    const int b = blk.z;
     if (ix >= g.X1 - 1 || iz >= g.Z1 - 1) return;
     const int64_t off = int64_t(b) * g.n, i = int64_t(iz) * g.nx + ix, nx = g.nx;
     const float *vx = s.f[VX] + off, *vz = s.f[VZ] + off;
@@ -176,9 +177,10 @@ __global__ void __launch_bounds__(256) vel_nopml_kernel(Geo g, SCoef c, Fields s
                                                         float *__restrict__ cvx, float *__restrict__ cvz)
 {
     const int M = MS > 0 ? MS : g.M;
-    const int ix = g.X0 + 1 + int(blockIdx.x) * 64 + int(threadIdx.x);
-    const int iz = g.Z0 + 1 + int(blockIdx.y) * 4 + int(threadIdx.y);
-    const int b = blockIdx.z;
+    const BlockXYZ blk = xcd_block();
+    const int ix = g.X0 + 1 + blk.x * 64 + int(threadIdx.x);
+    const int iz = g.Z0 + 1 + blk.y * 4 + int(threadIdx.y);
+    const int b = blk.z;
     if (ix >= g.X1 - 1 || iz >= g.Z1 - 1) return;
     const int64_t off = int64_t(b) * g.n, i = int64_t(iz) * g.nx + ix, nx = g.nx;
     const float *sxx = s.f[SXX] + off, *szz = s.f[SZZ] + off, *sxz = s.f[SXZ] + off;
@@ -212,9 +214,10 @@ template <int MS>
 __global__ void __launch_bounds__(256) vel_kernel(Geo g, SCoef c, Prof pr, Fields s, const float *__restrict__ rhob)
 {
     const int M = MS > 0 ? MS : g.M;
-    const int ix = M + int(blockIdx.x) * 64 + int(threadIdx.x);
-    const int iz = M + int(blockIdx.y) * 4 + int(threadIdx.y);
-    const int b = blockIdx.z;
+    const BlockXYZ blk = xcd_block();
+    const int ix = M + blk.x * 64 + int(threadIdx.x);
+    const int iz = M + blk.y * 4 + int(threadIdx.y);
+    const int b = blk.z;
     if (ix >= g.nx - M || iz >= g.nz - M) return;
     const int64_t off = int64_t(b) * g.n, i = int64_t(iz) * g.nx + ix, nx = g.nx;
     const float *sxx = s.f[SXX] + off, *szz = s.f[SZZ] + off, *sxz = s.f[SXZ] + off;
@@ -249,9 +252,10 @@ __global__ void __launch_bounds__(256) stress_kernel(Geo g, SCoef c, Prof pr, Fi
                                                      const float *__restrict__ mub, float *__restrict__ vcap)
 {
     const int M = MS > 0 ? MS : g.M;
-    const int ix = M + int(blockIdx.x) * 64 + int(threadIdx.x);
-    const int iz = M + int(blockIdx.y) * 4 + int(threadIdx.y);
-    const int b = blockIdx.z;
+    const BlockXYZ blk = xcd_block();
+    const int ix = M + blk.x * 64 + int(threadIdx.x);
+    const int iz = M + blk.y * 4 + int(threadIdx.y);
+    const int b = blk.z;
     if (ix >= g.nx - M || iz >= g.nz - M) return;
     const int64_t off = int64_t(b) * g.n, i = int64_t(iz) * g.nx + ix, nx = g.nx;
     const float *vx = s.f[VX] + off, *vz = s.f[VZ] + off;
