@@ -271,10 +271,13 @@ MEASURED_TRAFFIC = {
     ("acoustic3d", "bs", 2): {0: 8.812e9, 1: 10.325e9, 2: 4.770e9},
     # profiles/acoustic2d_384x1152_32shots_bs.md: adj_band 359.9 MB + adj_int 258.2 MB
     ("acoustic2d", "bs", 32): {1: 618.1e6},
+    # profiles/elastic2d_512x1024_16shots_bs.md: velocity + stress; prepare + apply; imaging
+    ("elastic2d", "bs", 16): {0: 982.9e6, 1: 2677.6e6, 2: 875.9e6},
 }
 TRAFFIC_SOURCE = {
     ("acoustic3d", "bs", 2): "profiles/acoustic3d_256x512x512_2per_launch_bs.md",
     ("acoustic2d", "bs", 32): "profiles/acoustic2d_384x1152_32shots_bs.md",
+    ("elastic2d", "bs", 16): "profiles/elastic2d_512x1024_16shots_bs.md",
 }
 TILE_ENV = ("SWP_TZ", "SWP_TZ_BAND", "SWP_TZ_RECON", "SWP_TZ3", "SWP_REGION_WAVES", "SWP_KERNELS", "SWP_SHELL3")
 

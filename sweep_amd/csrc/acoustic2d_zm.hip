@@ -310,7 +310,9 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
         const float v = vp[r];
         q[k] = v * v * dt2 * ln[r];
     }
-    float n_v, n_l, n_hv, n_hl, n_lp, n_im = 0.f;
+    // row z+M of the queue fields, row z's halo, lambda_prev and lambda_now
+    // (the centre, used unscaled) in flight
+    float n_v, n_l, n_hv, n_hl, n_lp, n_gun, n_im = 0.f, n_v0 = 0.f, n_gr = 0.f;
     auto issue = [&](int zr) {
         const int64_t rn = int64_t(zr) * px;
         n_v = vp[rn + int64_t(M) * px + xc];
@@ -318,14 +320,19 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
         n_hv = vp[rn + h.col];
         n_hl = ln[rn + h.col];
         n_lp = lp[rn + xc];
-        if constexpr (IMG) n_im = img[off + rn + xc];
+        n_gun = ln[rn + xc];
+        if constexpr (IMG) {   // the imaging operands, the gradient one row ahead (own lane only)
+            n_im = img[off + rn + xc];
+            n_v0 = vp[rn + xc];
+            n_gr = grad[off + rn + xc];
+        }
     };
     issue(zb);
     for (int z = zb; z < ze; ++z) {
         const int64_t rz = int64_t(z) * px;
         q[2 * M] = n_v * n_v * dt2 * n_l;
         const float hv = n_hv * n_hv * dt2 * n_hl;
-        const float lpv = n_lp, im = n_im;
+        const float lpv = n_lp, gun = n_gun, im = n_im, v0 = n_v0, gr = n_gr;
         issue(min(z + 1, ze - 1));
         WAVE_SYNC();
         row[M + lane] = q[M];
@@ -339,8 +346,7 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
             lapz += c.lap[k] * ((q[M + k] - gw) + (q[M - k] - gw));
         }
         if (active) {
-            const float gun = ln[rz + x];
-            if constexpr (IMG) grad[off + rz + x] += 2.f * g.dt * g.dt * im * gun / vp[rz + x];
+            if constexpr (IMG) grad[off + rz + x] = gr + 2.f * g.dt * g.dt * im * gun / v0;
             lnx[rz + x] = 2.0f * gun - lpv + lapx * g.inv_dx2.inv + lapz * g.inv_dz2.inv;
         }
 #pragma unroll
@@ -411,7 +417,7 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
     }
     // row z+M (queue tail) and row z (halo, own x quantities) in flight
     float n_v, n_l, n_zz = 0.f, n_pz = 0.f, n_hv, n_hl, n_hz = 0.f, n_hp = 0.f;
-    float n_gun, n_lp, n_v0, n_zx = 0.f, n_px = 0.f, n_im = 0.f;
+    float n_gun, n_lp, n_v0, n_zx = 0.f, n_px = 0.f, n_im = 0.f, n_zzc = 0.f, n_pzc = 0.f, n_gr = 0.f;
     auto issue = [&](int zr) {
         const int64_t rq = int64_t(zr + M) * px + xc;
         n_v = vp[rq];
@@ -424,7 +430,10 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
         n_gun = ln[rn + xc];
         n_lp = lp[rn + xc];
         n_v0 = vp[rn + xc];
-        if constexpr (IMG) n_im = img[off + rn + xc];
+        if constexpr (IMG) {   // the gradient one row ahead (own lane only)
+            n_im = img[off + rn + xc];
+            n_gr = grad[off + rn + xc];
+        }
     };
     issue(zb);
     for (int z = zb; z < ze; ++z) {
@@ -432,6 +441,7 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
         put_row(z + M, 2 * M, n_v, n_l, n_zz, n_pz);
         const float hvv = n_hv, hl = n_hl, hz = n_hz, hpx = n_hp;
         const float gun = n_gun, lpv = n_lp, v0 = n_v0, zx = n_zx, px = n_px, im = n_im;
+        const float zzc = n_zzc, pzc = n_pzc, gr = n_gr;
         issue(min(z + 1, ze - 1));
         const float hcc = hvv * hvv * dt2;
         const float hg = hcc * hl;
@@ -453,7 +463,7 @@ __global__ void __launch_bounds__(64) adj_band_kernel(Geo g, Coef c, Prof pr, Ad
         }
         WAVE_SYNC();
         if (IMG && active && x >= g.X0 && x < g.X1 && z >= g.Z0 && z < g.Z1)
-            grad[off + rz + x] += 2.f * g.dt * g.dt * im * gun / v0;
+            grad[off + rz + x] = gr + 2.f * g.dt * g.dt * im * gun / v0;
         const float gw = cc0 * gun;
         const bool zin = z >= pz0 && z < pz1;
         if (xin && zin) {
@@ -560,20 +570,25 @@ __global__ void __launch_bounds__(64) recon_zm_kernel(Geo g, Coef c, const float
     float q[2 * M + 1];
 #pragma unroll
     for (int k = 0; k < 2 * M; ++k) q[k] = un[int64_t(zb - M + k) * px + xc];
-    float n_q, n_h, n_up, n_v, n_a = 0.f;
+    // the gradient is read with the row's other fields, one row ahead: each
+    // cell's gradient is updated by its own lane only
+    float n_q, n_h, n_up, n_v, n_a = 0.f, n_gr = 0.f;
     auto issue = [&](int zr) {
         const int64_t rn = int64_t(zr) * px;
         n_q = un[rn + int64_t(M) * px + xc];
         n_h = un[rn + h.col];
         n_up = up[rn + xc];
         n_v = vp[rn + xc];
-        if (IMG) n_a = adj[off + rn + xc];
+        if (IMG) {
+            n_a = adj[off + rn + xc];
+            n_gr = grad[off + rn + xc];
+        }
     };
     issue(zb);
     for (int z = zb; z < ze; ++z) {
         const int64_t rz = int64_t(z) * px;
         q[2 * M] = n_q;
-        const float hv = n_h, upv = n_up, v = n_v, a = n_a;
+        const float hv = n_h, upv = n_up, v = n_v, a = n_a, gr = n_gr;
         issue(min(z + 1, ze - 1));
         const bool img_row = IMG && z >= g.Z0 + g.W && z < g.Z1 - g.W;
         WAVE_SYNC();
@@ -588,7 +603,7 @@ __global__ void __launch_bounds__(64) recon_zm_kernel(Geo g, Coef c, const float
             u_next[off + rz + x] = unx;
             if (img_row && ximg) {
                 const float u_tt = fd::scaled(unx - 2 * u0 + upv, g.inv_dt2);
-                grad[off + rz + x] += 2.f * dt * dt * u_tt * a / v;
+                grad[off + rz + x] = gr + 2.f * dt * dt * u_tt * a / v;
             }
         }
 #pragma unroll
