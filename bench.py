@@ -269,8 +269,8 @@ REF_SAMPLE_STEPS = {"acoustic3d": 8, "acoustic2d": 1000, "elastic2d": 300, "acou
 MEASURED_TRAFFIC = {
     # profiles/acoustic3d_256x512x512_2per_launch_bs.md: the eight region launches of one step
     ("acoustic3d", "bs", 2): {0: 8.812e9, 1: 10.325e9, 2: 4.770e9},
-    # profiles/acoustic2d_384x1152_32shots_bs.md: adj_band 359.9 MB + adj_int 258.2 MB
-    ("acoustic2d", "bs", 32): {1: 618.1e6},
+    # profiles/acoustic2d_384x1152_32shots_bs.md: forward, adjoint (band + interior) and reconstruction
+    ("acoustic2d", "bs", 32): {0: 462.7e6, 1: 524.3e6, 2: 433.6e6},
     # profiles/elastic2d_512x1024_16shots_bs.md: velocity + stress; prepare + apply; imaging
     ("elastic2d", "bs", 16): {0: 982.9e6, 1: 2677.6e6, 2: 875.9e6},
 }
