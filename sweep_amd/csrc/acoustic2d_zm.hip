@@ -100,23 +100,25 @@ __global__ void __launch_bounds__(64) fwd_int_kernel(Geo g, Coef c, StepFields f
     float q[2 * M + 1];
 #pragma unroll
     for (int k = 0; k < 2 * M; ++k) q[k] = un[int64_t(zb - M + k) * px + xc];
-    // row zb in flight
-    float n_q = un[int64_t(zb + M) * px + xc];
-    float n_h = un[int64_t(zb) * px + h.col];
-    float n_up = up[int64_t(zb) * px + xc];
-    float n_v = vp[int64_t(zb) * px + xc];
-    for (int z = zb; z < ze; ++z) {
+    // Two rows of loads in flight: slots A and B alternate, each refilled with
+    // row z+2 as soon as row z has taken its values, so a row's loads have two
+    // rows of compute to land (the kernel is bound by load latency; one slot
+    // reused by register copies would make each row wait for the next).
+    struct Ahead {
+        float q, h, up, v;
+    };
+    auto fetch = [&](int zr, Ahead &n) {   // clamped on the last row
+        const int64_t rn = int64_t(min(zr, ze - 1)) * px;
+        n.q = un[rn + int64_t(M) * px + xc];
+        n.h = un[rn + h.col];
+        n.up = up[rn + xc];
+        n.v = vp[rn + xc];
+    };
+    auto step = [&](int z, Ahead &n) {
         const int64_t rz = int64_t(z) * px;
-        q[2 * M] = n_q;
-        const float hv = n_h, upv = n_up, v = n_v;
-        {   // issue row z+1 (clamped on the last row)
-            const int zn = min(z + 1, ze - 1);
-            const int64_t rn = int64_t(zn) * px;
-            n_q = un[rn + int64_t(M) * px + xc];
-            n_h = un[rn + h.col];
-            n_up = up[rn + xc];
-            n_v = vp[rn + xc];
-        }
+        q[2 * M] = n.q;
+        const float hv = n.h, upv = n.up, v = n.v;
+        fetch(z + 2, n);
         WAVE_SYNC();
         row[M + lane] = q[M];
         if (h.on) row[h.slot] = hv;
@@ -130,6 +132,13 @@ __global__ void __launch_bounds__(64) fwd_int_kernel(Geo g, Coef c, StepFields f
         }
 #pragma unroll
         for (int k = 0; k < 2 * M; ++k) q[k] = q[k + 1];
+    };
+    Ahead A, B;
+    fetch(zb, A);
+    fetch(zb + 1, B);
+    for (int z = zb; z < ze; z += 2) {
+        step(z, A);
+        if (z + 1 < ze) step(z + 1, B);
     }
 }
 
@@ -310,30 +319,33 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
         const float v = vp[r];
         q[k] = v * v * dt2 * ln[r];
     }
-    // row z+M of the queue fields, row z's halo, lambda_prev and lambda_now
-    // (the centre, used unscaled) in flight
-    float n_v, n_l, n_hv, n_hl, n_lp, n_gun, n_im = 0.f, n_v0 = 0.f, n_gr = 0.f;
-    auto issue = [&](int zr) {
-        const int64_t rn = int64_t(zr) * px;
-        n_v = vp[rn + int64_t(M) * px + xc];
-        n_l = ln[rn + int64_t(M) * px + xc];
-        n_hv = vp[rn + h.col];
-        n_hl = ln[rn + h.col];
-        n_lp = lp[rn + xc];
-        n_gun = ln[rn + xc];
-        if constexpr (IMG) {   // the imaging operands, the gradient one row ahead (own lane only)
-            n_im = img[off + rn + xc];
-            n_v0 = vp[rn + xc];
-            n_gr = grad[off + rn + xc];
+    // Rows in flight, two slots as in fwd_int_kernel: row z+M of the queue
+    // fields, row z's halo, lambda_prev and lambda_now (the centre, used
+    // unscaled), and with IMG the imaging operands and the gradient (each
+    // cell's gradient is updated by its own lane only).
+    struct Ahead {
+        float v, l, hv, hl, lp, gun, im, v0, gr;
+    };
+    auto fetch = [&](int zr, Ahead &n) {   // clamped on the last row
+        const int64_t rn = int64_t(min(zr, ze - 1)) * px;
+        n.v = vp[rn + int64_t(M) * px + xc];
+        n.l = ln[rn + int64_t(M) * px + xc];
+        n.hv = vp[rn + h.col];
+        n.hl = ln[rn + h.col];
+        n.lp = lp[rn + xc];
+        n.gun = ln[rn + xc];
+        if constexpr (IMG) {
+            n.im = img[off + rn + xc];
+            n.v0 = vp[rn + xc];
+            n.gr = grad[off + rn + xc];
         }
     };
-    issue(zb);
-    for (int z = zb; z < ze; ++z) {
+    auto step = [&](int z, Ahead &n) {
         const int64_t rz = int64_t(z) * px;
-        q[2 * M] = n_v * n_v * dt2 * n_l;
-        const float hv = n_hv * n_hv * dt2 * n_hl;
-        const float lpv = n_lp, gun = n_gun, im = n_im, v0 = n_v0, gr = n_gr;
-        issue(min(z + 1, ze - 1));
+        q[2 * M] = n.v * n.v * dt2 * n.l;
+        const float hv = n.hv * n.hv * dt2 * n.hl;
+        const float lpv = n.lp, gun = n.gun, im = n.im, v0 = n.v0, gr = n.gr;
+        fetch(z + 2, n);
         WAVE_SYNC();
         row[M + lane] = q[M];
         if (h.on) row[h.slot] = hv;
@@ -351,6 +363,13 @@ __global__ void __launch_bounds__(64) adj_int_kernel(Geo g, Coef c, AdjFields f,
         }
 #pragma unroll
         for (int k = 0; k < 2 * M; ++k) q[k] = q[k + 1];
+    };
+    Ahead A{}, B{};
+    fetch(zb, A);
+    fetch(zb + 1, B);
+    for (int z = zb; z < ze; z += 2) {
+        step(z, A);
+        if (z + 1 < ze) step(z + 1, B);
     }
 }
 
