@@ -62,7 +62,9 @@ WORKLOADS = {
     # (x 96, z 48), gradient of 0.5 ||pred - obs||^2 from a constant 1500 m/s start
     "acoustic2d_small": dict(name="acoustic2d_96x128_nt800_1shot_bs", shape=(96, 128), dh=10.0, dt=2e-3, nt=800,
                              order=8, abcn=50, nshots=1, f0=10.0, delay=0.14, seed=0, per_launch=1,
-                             two_layer=True, src=(32, 48), rec=(96, 48)),
+                             two_layer=True, src=(32, 48), rec=(96, 48),
+                             note="the reference README's case at order 8 with boundary saving; the README "
+                                  "runs Acoustic() at its default order 4 without checkpointing"),
     "elastic2d": dict(name="elastic2d_512x1024_nt4000_16shots_bs", shape=(512, 1024), dh=5.0, dt=5e-4,
                       nt=4000, order=8, abcn=50, nshots=16, f0=12.0, seed=6, per_launch=16, nrec=400),
 }
@@ -433,6 +435,7 @@ def main():
                    "memory": args.memory or "bs",
                    "tape_bytes_per_launch": tape_bytes,
                    "graphs": args.graphs,
+                   **({"note": W["note"]} if "note" in W else {}),
                    **({"autotuned": autotuned} if autotuned else {})},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
