@@ -17,7 +17,7 @@ One iteration over the shot batches:
    and the model stays.
 
 The reference ships no FWI driver (its examples build one in notebooks);
-this one is the plain method against which a learned step is judged: a
+this one is the plain method against which a learned step is measured: a
 learned update is accepted only where it lowers the same misfit.
 """
 from __future__ import annotations

@@ -2,7 +2,7 @@
 [gradient, source illumination, current vp] to a model update, trained on
 pairs the solver itself generates; the plain illumination-preconditioned
 line-search step (``sweep_amd.fwi.PlainFWI``) is both the fallback and the
-judge.
+arbiter (a network step is kept only if the solver's misfit falls).
 
 * Pairs: for seeded random layered models, plain FWI runs from a smoothed
   start; at each iteration the input planes (g, S, vp) and the target

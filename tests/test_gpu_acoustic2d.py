@@ -1,7 +1,7 @@
 """GPU parity of the 2-D acoustic path: libswp_core through PropTorch against
 the oracle (same inputs, same memory mode) and the reference's golden vectors.
 
-Bars (north star): records rel-L2 <= 1e-5 and model gradients rel-L2 <= 1e-4
+Bars: records rel-L2 <= 1e-5 and model gradients rel-L2 <= 1e-4
 against the oracle; CPML profiles and index maps bit-exact (checked on the
 host in test_tables.py / test_host.py).  Against the reference's eager golden
 vectors the bar is the reference's own compiled-vs-eager tolerance (1e-3,
