@@ -6,7 +6,8 @@
 // halo each side) in its own LDS row for the x-taps.  Per cell that is one
 // global load per field read and one store per field written.  The loads of
 // row z+1 are issued before row z is computed (software pipelining), so each
-// wave keeps a row of loads in flight while it computes.  The LDS row is
+// wave keeps a row of loads in flight while it computes; the interior
+// kernels, with registers to spare, keep two (rows z+1 and z+2).  The LDS row is
 // private to the wave; a workgroup barrier (one wave: no waiting on other
 // waves) with its LDS fence orders each row's stores against the reads of
 // the row before and after -- a wave-level scheduling barrier alone does not
